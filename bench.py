@@ -1,0 +1,156 @@
+"""Benchmark of the lattice hot path (one leapfrog step = kick + drift + wrap).
+
+Workload: the 256^3 FP64 drop-in configuration (dx = 0.5, dt = 0.05,
+V = phi^2/2 - phi^4/4 + 0.2 phi^6/6 in the reference's units, 10^4 steps with
+energy outputs every 200 steps).  `value` is FP64 lattice site-updates per
+second with the fields resident in HBM; the step kernel is timed with HIP
+events on the stream it runs on.  Beside it the line reports the FP32 form
+(FP32 fields, FP64 compensated energy sums) with its energy difference to the
+FP64 run, the 512^3 FP64 roofline case, and the CPU baseline.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "site-updates/s & % HBM roofline, FP32-comp vs FP64 at dE/E<=1e-6"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def model(lam=1.0, g=0.2):
+    L = 6.0 * lam
+    return dict(a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1, kappa=float(np.sqrt(L * L / (120.0 * g))),
+                phi0=float(np.sqrt(L)) * 1.0)
+
+
+def synthetic_fields(N, phi0, seed=1234):
+    rng = np.random.default_rng(seed)
+    phi = phi0 + 0.05 * np.sqrt(6.0) * rng.standard_normal(N ** 3)
+    pi = 0.05 * np.sqrt(6.0) * rng.standard_normal(N ** 3)
+    return phi, pi
+
+
+def run_leg(pn, N, precision, steps, warmup, out_every, phi, pi, m):
+    """Returns (total event ms over `steps`, energies at each output)."""
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)
+    energies = []
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
+                    precision=precision) as L:
+        L.set_fields(phi, pi)
+        if warmup:
+            L.evolve(dtk[:warmup], dtd[:warmup])
+        energies.append(L.volume_averages().energy)
+        total = 0.0
+        s = warmup
+        while s < warmup + steps:
+            n = min(out_every, warmup + steps - s)
+            total += L.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+            s += n
+            energies.append(L.volume_averages().energy)
+    return total, np.array(energies)
+
+
+def cpu_baseline(N, m, target_s=12.0):
+    import ctypes as C
+    path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.cb_run.restype = C.c_double
+    lib.cb_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                           C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    ck = C.c_double()
+    probe = lib.cb_run(N, threads, 1, 3, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
+                       m["kappa"], 0.1, m["phi0"], C.byref(ck))
+    steps = max(3, int(target_s / max(probe / 3.0, 1e-6)))
+    sec = lib.cb_run(N, threads, 2, steps, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
+                     m["kappa"], 0.1, m["phi0"], C.byref(ck))
+    return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d^3 FP64 leapfrog, %d timed steps (%.1f s) of the reference loop nest "
+                      "(AoS + z halo, pos()/mod() per site, kick then drift, z slabs over %d threads), "
+                      "reference flags -O3 -ffast-math" % (N, steps, sec, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--out-every", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the lattice fits one MI355X; "
+              "multi-GPU z-slab sharding is a design, not measured here)", file=sys.stderr)
+        sys.exit(2)
+
+    import phinum_amd as pn
+
+    N, K, W = args.size, args.steps, args.warmup
+    m = model()
+    phi, pi = synthetic_fields(N, m["phi0"])
+    log("fp64 %d^3: %d warmup + %d timed steps" % (N, W, K))
+    t0 = time.time()
+    ms64, e64 = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, phi, pi, m)
+    log("fp64 done in %.1fs wall, %.3f ms/step" % (time.time() - t0, ms64 / K))
+    sites = float(N) ** 3
+    value = sites * K / (ms64 * 1e-3)
+    ms_step = ms64 / K
+    achieved = 32.0 * sites / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "fp64 leapfrog %d^3, dx=0.5, dt=0.05, lambda=1, g=0.2, %d steps, "
+                               "energy every %d steps" % (N, K, args.out_every),
+                   "N": N, "out_every": args.out_every},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step<double> (fused kick+drift)", "bytes_per_site": 32},
+        "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
+    }
+    if not args.no_extra:
+        log("fp32 %d^3" % N)
+        ms32, e32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, phi, pi, m)
+        v32 = sites * K / (ms32 * 1e-3)
+        line["fp32_comp"] = {
+            "value": v32, "unit": "site-updates/s", "ms_per_step": ms32 / K,
+            "ratio_vs_fp64": v32 / value,
+            "dE_over_E_vs_fp64_max": float(np.max(np.abs(e32 - e64) / np.abs(e64))),
+            "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
+            "roofline_frac": (16.0 * sites / (ms32 / K * 1e-3) / 1e9) / HBM_PEAK_GBS,
+        }
+        N5 = 512
+        log("fp64 %d^3 roofline case" % N5)
+        p5, q5 = synthetic_fields(N5, m["phi0"], seed=5)
+        ms5, _ = run_leg(pn, N5, pn.PN_FP64, 200, 20, 200, p5, q5, m)
+        a5 = 32.0 * float(N5) ** 3 / (ms5 / 200 * 1e-3) / 1e9
+        line["roofline_512"] = {"bound": "hbm", "achieved": a5, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": a5 / HBM_PEAK_GBS, "ms_per_step": ms5 / 200,
+                                "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3)}
+    if not args.no_cpu:
+        log("cpu baseline")
+        cb = cpu_baseline(N, m)
+        if cb:
+            line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * phinum_gpu.h -- C ABI of the MI355X (gfx950) lattice hot path of PhiNum.
+ *
+ * Plain C types only (no torch, no HIP types).  One context owns all device
+ * memory, FFT plans and one HIP stream; the caller owns every host array and
+ * every output struct.  Every call returns PN_OK (0) or a negative PN_ERR_*
+ * code and leaves a message for pn_last_error().  Calls are synchronous at
+ * return unless the name says otherwise.  One context per host thread.
+ *
+ * Lattice conventions follow the reference (one scalar component, Nc = 1):
+ *   pos(x,y,z) = x + N*y + N*N*z                      (util.cpp:34-37)
+ *   PN_LAYOUT_REF   : N*N*(N+2) values, interior planes z = 1..N plus the two
+ *                     z halo planes 0 and N+1          (lattice.cpp:70-104)
+ *   PN_LAYOUT_DENSE : N*N*N values, interior only, same x-fastest order
+ * On the device the fields are stored dense (structure of arrays), and the
+ * periodic wrap in x, y and z is index arithmetic inside the kernels.
+ * Site indices returned by the library are reference pos() indices
+ * (PN_LAYOUT_REF numbering) so they can be compared with the reference.
+ */
+#ifndef PHINUM_GPU_H
+#define PHINUM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PN_ABI_VERSION 1
+
+#define PN_OK 0
+#define PN_ERR_ARG (-1)      /* bad argument or size                        */
+#define PN_ERR_HIP (-2)      /* HIP runtime error (message has the HIP text) */
+#define PN_ERR_FFT (-3)      /* rocFFT error                                */
+#define PN_ERR_STATE (-4)    /* call not valid in the context's state       */
+#define PN_ERR_NOMEM (-5)    /* host or device allocation failed            */
+#define PN_ERR_CAPACITY (-6) /* caller-provided output array too small      */
+
+enum pn_precision { PN_FP64 = 0, PN_FP32 = 1 };
+enum pn_layout { PN_LAYOUT_REF = 0, PN_LAYOUT_DENSE = 1 };
+
+/* Theory 0 (phi^4-phi^6), no expansion.  Replaces the compile-time switches
+ * and mutable globals of parameters.cpp:84-100,184-201 and the bare mass of
+ * renormalization.cpp:3-31 (BareM_sqr_long = M_sqr). */
+typedef struct pn_params {
+  int N;           /* N_t = N_eta, lattice extent (>= 4)                     */
+  double a;        /* a_t = a_eta, lattice spacing                           */
+  int mass_mode;   /* Mass_Mode (+1, 0, -1)                                  */
+  double m2;       /* BareM_sqr_long                                         */
+  int lambda_mode; /* Lambda_Mode (+1, 0, -1)                                */
+  double kappa;    /* Kappa                                                  */
+  int precision;   /* enum pn_precision: FP64, or FP32 fields with FP64 sums */
+  int device;      /* HIP device ordinal                                     */
+} pn_params;
+
+typedef struct pn_ctx pn_ctx;
+
+/* Volume averages, fieldanalysis.cpp:399-699 (LocalLattice::avg*). */
+typedef struct pn_averages {
+  double phi, pi, phi2, pi2, phipi, phi4, phi6;
+  double energy, interaction, charge;
+} pn_averages;
+
+/* Extrema, fieldanalysis.cpp:319-397 (GlminPhi, GlmaxPhi, GlmaxAbsPhi and the
+ * first raster position of each, as pos() indices; -1 when never set). */
+typedef struct pn_extrema {
+  double min, max, maxabs;
+  long long minpos, maxpos, maxabspos;
+} pn_extrema;
+
+/* Binned shell spectrum, momentumspace.cpp:140-278 (TotalCorrF/K/Q and
+ * TotalBCounts).  Arrays are caller-allocated with pn_spectrum_nbins()
+ * entries.  binsize is pTmin (momentumspace.cpp:40-42). */
+typedef struct pn_spectrum {
+  int nbins;
+  double binsize;
+  int *counts;
+  double *corr_f; /* sum |phi_k|^2 per shell   */
+  double *corr_k; /* sum |pi_k|^2 per shell    */
+  double *corr_q; /* sum Re(phi_k pi_k^*)      */
+} pn_spectrum;
+
+/* One island (islands.cpp:13-28, filled as BFS does at :240-264).  Islands
+ * are numbered in the order of their smallest site index, which is the order
+ * in which the reference's raster scan (islands.cpp:827-835) discovers them
+ * when no islands are tracked yet. */
+typedef struct pn_island {
+  long long index;      /* island number (0,1,2,... in discovery order)     */
+  long long nn;         /* number of sites                                  */
+  long long first_pos;  /* smallest pos() index of the island               */
+  long long peak_pos;   /* pos() index of the peak (max phi^2, or Delta)    */
+  double volume;        /* nn * a^3                                         */
+  double peak_value;    /* Phi (mode 0) or Delta (mode 1) at the peak       */
+  double avg_phi;       /* sum phi / nn                                     */
+  double pot_en, kin_en, grad_en;  /* each times a^3                        */
+  long long sum_dx, sum_dy, sum_dz;  /* sum of site offsets from the peak:
+                           x, y minimum-image (periodic), z plain (open)     */
+  double cx, cy, cz;    /* centroid = peak + sum_d / nn (x, y wrapped)      */
+  long long peak_ties;  /* sites (besides the peak) with the same peak value */
+} pn_island;
+
+typedef struct pn_catalog {
+  int capacity;        /* entries available in islands[]                    */
+  int count;           /* islands found (may exceed capacity: PN_ERR_CAPACITY) */
+  double threshold;    /* threshold used                                    */
+  pn_island *islands;
+  int *labels;         /* optional, N^3 dense: island index or -1           */
+} pn_catalog;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* LocalLattice::init (lattice.cpp:70-104). */
+int pn_create(const pn_params *p, pn_ctx **out);
+/* LocalLattice::fin (lattice.cpp:108-117). */
+int pn_destroy(pn_ctx *c);
+/* Text of the last error on this context (or of the last failed pn_create
+ * in this thread when c is NULL). */
+const char *pn_last_error(const pn_ctx *c);
+int pn_abi_version(void);
+
+/* ---- fields ----------------------------------------------------------- */
+
+/* Host -> device copy of Phi and Pi (either may be NULL to keep the device
+ * copy).  Replaces direct writes to LocalLattice::Phi/Pi followed by
+ * Communication::exchange (initialconditions.cpp:636-703).  FP32 contexts
+ * round to nearest. */
+int pn_set_fields(pn_ctx *c, const double *phi, const double *pi, int layout);
+/* Device -> host copy.  PN_LAYOUT_REF also fills the halo planes as
+ * Communication::exchange leaves them (comm.cpp:63-78). */
+int pn_get_fields(pn_ctx *c, double *phi, double *pi, int layout);
+
+/* ---- dynamics (dynamics.cpp) ------------------------------------------ */
+
+/* updateMomenta (dynamics.cpp:52-175) with dtk = tau - Last_Momenta_update. */
+int pn_kick(pn_ctx *c, double dtk);
+/* updateFields (dynamics.cpp:177-194) with dtd = tau - Last_Field_update. */
+int pn_drift(pn_ctx *c, double dtd);
+/* evolve() body (dynamics.cpp:196-237): kick, drift and the periodic wrap of
+ * Communication::exchange fused into one launch per step.  dtk[n], dtd[n]
+ * are the host-replayed factors of step n.  Enqueued; pn_sync waits. */
+int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double *dtd);
+int pn_sync(pn_ctx *c);
+/* pn_evolve bracketed by HIP events on the context's stream; waits for the
+ * steps and returns their device time in *ms (measurement helper, no
+ * reference counterpart). */
+int pn_evolve_timed(pn_ctx *c, int nsteps, const double *dtk, const double *dtd, double *ms);
+
+/* ---- analysis --------------------------------------------------------- */
+
+/* CalculateVolumeAverages (fieldanalysis.cpp:399-699), FP64 compensated sums. */
+int pn_volume_averages(pn_ctx *c, pn_averages *out);
+/* CalculateExtrema (fieldanalysis.cpp:319-397). */
+int pn_extrema_get(pn_ctx *c, pn_extrema *out);
+/* nTBins of momentumspace.cpp:40-47 for this N and a. */
+int pn_spectrum_nbins(int N, double a);
+/* Calculate_CorrelationFunctions (momentumspace.cpp:140-278) with
+ * fftXtoP_direct (clusterfft.cpp:185-225): rocFFT r2c of Phi and Pi, times
+ * a^3, binned by a hand-written kernel. */
+int pn_spectrum_get(pn_ctx *c, pn_spectrum *out);
+/* fftXtoP_direct (clusterfft.cpp:185-225): the a^3-scaled r2c transform of
+ * Phi (field 0) or Pi (field 1) as interleaved complex values in the
+ * reference's p-space order py + px*(N/2+1) + pz*N*(N/2+1), N*N*(N/2+1)
+ * modes (2x that many doubles). */
+int pn_fft_modes(pn_ctx *c, int field, double *out);
+/* Island catalog: is_island + BFS + the raster scan of Full_Analysis
+ * (islands.cpp:72-279,788-835) as GPU connected-component labelling.
+ * mode 0: phi^2/2 + pi^2/2 > thr^2/2 (islands.cpp:76); peak = max phi^2.
+ * mode 1: Delta > thr (islands.cpp:83) with Delta of fieldanalysis.cpp:447,514-515;
+ *         peak = max Delta.
+ * Connectivity: 6 neighbours, periodic in x and y, open in z (islands.cpp:200-236). */
+int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out);
+
+/* Energy density per site rho = pi^2/2 + (1/2) sum_i (grad+_i phi)^2 + V(phi),
+ * the summand of Energy (fieldanalysis.cpp:625), dense order. */
+int pn_energy_density(pn_ctx *c, double *rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHINUM_GPU_H */

@@ -1,0 +1,268 @@
+"""phinum_amd -- MI355X (gfx950) lattice hot path of PhiNum.
+
+Python side of the C ABI declared in include/phinum_gpu.h (ctypes).  The
+shared library libphinum_gpu.so is built in-tree by phinum_amd/build.py; if
+it is missing, importing the bindings raises instead of falling back to any
+CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphinum_gpu.so")
+
+PN_OK = 0
+PN_ERR_ARG = -1
+PN_ERR_HIP = -2
+PN_ERR_FFT = -3
+PN_ERR_STATE = -4
+PN_ERR_NOMEM = -5
+PN_ERR_CAPACITY = -6
+PN_FP64, PN_FP32 = 0, 1
+PN_LAYOUT_REF, PN_LAYOUT_DENSE = 0, 1
+
+
+class PnParams(C.Structure):
+    _fields_ = [("N", C.c_int), ("a", C.c_double), ("mass_mode", C.c_int), ("m2", C.c_double),
+                ("lambda_mode", C.c_int), ("kappa", C.c_double), ("precision", C.c_int),
+                ("device", C.c_int)]
+
+
+class PnAverages(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("phi", "pi", "phi2", "pi2", "phipi", "phi4", "phi6",
+                                          "energy", "interaction", "charge")]
+
+    def as_array(self):
+        return np.array([getattr(self, n) for n, _ in self._fields_])
+
+
+class PnExtrema(C.Structure):
+    _fields_ = [("min", C.c_double), ("max", C.c_double), ("maxabs", C.c_double),
+                ("minpos", C.c_longlong), ("maxpos", C.c_longlong), ("maxabspos", C.c_longlong)]
+
+
+class PnSpectrum(C.Structure):
+    _fields_ = [("nbins", C.c_int), ("binsize", C.c_double), ("counts", C.POINTER(C.c_int)),
+                ("corr_f", C.POINTER(C.c_double)), ("corr_k", C.POINTER(C.c_double)),
+                ("corr_q", C.POINTER(C.c_double))]
+
+
+class PnIsland(C.Structure):
+    _fields_ = [("index", C.c_longlong), ("nn", C.c_longlong), ("first_pos", C.c_longlong),
+                ("peak_pos", C.c_longlong), ("volume", C.c_double), ("peak_value", C.c_double),
+                ("avg_phi", C.c_double), ("pot_en", C.c_double), ("kin_en", C.c_double),
+                ("grad_en", C.c_double), ("sum_dx", C.c_longlong), ("sum_dy", C.c_longlong),
+                ("sum_dz", C.c_longlong), ("cx", C.c_double), ("cy", C.c_double),
+                ("cz", C.c_double), ("peak_ties", C.c_longlong)]
+
+
+class PnCatalog(C.Structure):
+    _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("threshold", C.c_double),
+                ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libphinum_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("phinum_amd: %s is missing; run phinum_amd/build.py (or "
+                           "__graft_entry__.build()) first" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp = P(C.c_double)
+    sig = {
+        "pn_create": (C.c_int, [P(PnParams), P(C.c_void_p)]),
+        "pn_destroy": (C.c_int, [C.c_void_p]),
+        "pn_last_error": (C.c_char_p, [C.c_void_p]),
+        "pn_abi_version": (C.c_int, []),
+        "pn_set_fields": (C.c_int, [C.c_void_p, dp, dp, C.c_int]),
+        "pn_get_fields": (C.c_int, [C.c_void_p, dp, dp, C.c_int]),
+        "pn_kick": (C.c_int, [C.c_void_p, C.c_double]),
+        "pn_drift": (C.c_int, [C.c_void_p, C.c_double]),
+        "pn_evolve": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+        "pn_sync": (C.c_int, [C.c_void_p]),
+        "pn_evolve_timed": (C.c_int, [C.c_void_p, C.c_int, dp, dp, dp]),
+        "pn_volume_averages": (C.c_int, [C.c_void_p, P(PnAverages)]),
+        "pn_extrema_get": (C.c_int, [C.c_void_p, P(PnExtrema)]),
+        "pn_spectrum_nbins": (C.c_int, [C.c_int, C.c_double]),
+        "pn_spectrum_get": (C.c_int, [C.c_void_p, P(PnSpectrum)]),
+        "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),
+        "pn_islands": (C.c_int, [C.c_void_p, C.c_int, C.c_double, P(PnCatalog)]),
+        "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class PhiNumError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("phinum_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class Lattice:
+    """One lattice context on one GPU (pn_create ... pn_destroy).
+
+    Mirrors LocalLattice (lattice.cpp:8-118) plus the hot-path functions of
+    dynamics.cpp, fieldanalysis.cpp, momentumspace.cpp and islands.cpp.
+    """
+
+    def __init__(self, N, a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1, kappa=2.0,
+                 precision=PN_FP64, device=0):
+        self.L = lib()
+        self.N = int(N)
+        self.a = float(a)
+        self.params = PnParams(int(N), float(a), int(mass_mode), float(m2), int(lambda_mode),
+                               float(kappa), int(precision), int(device))
+        h = C.c_void_p()
+        rc = self.L.pn_create(C.byref(self.params), C.byref(h))
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # fields ---------------------------------------------------------------
+    def set_fields(self, phi=None, pi=None, layout=PN_LAYOUT_DENSE):
+        phi = None if phi is None else np.ascontiguousarray(phi, dtype=np.float64)
+        pi = None if pi is None else np.ascontiguousarray(pi, dtype=np.float64)
+        n = self.N ** 3 if layout == PN_LAYOUT_DENSE else self.N ** 2 * (self.N + 2)
+        for arr in (phi, pi):
+            if arr is not None and arr.size != n:
+                raise ValueError("field has %d values, expected %d" % (arr.size, n))
+        self._chk(self.L.pn_set_fields(self.h, _dptr(phi), _dptr(pi), layout))
+
+    def get_fields(self, layout=PN_LAYOUT_DENSE):
+        n = self.N ** 3 if layout == PN_LAYOUT_DENSE else self.N ** 2 * (self.N + 2)
+        phi = np.empty(n)
+        pi = np.empty(n)
+        self._chk(self.L.pn_get_fields(self.h, _dptr(phi), _dptr(pi), layout))
+        return phi, pi
+
+    # dynamics -------------------------------------------------------------
+    def kick(self, dtk):
+        self._chk(self.L.pn_kick(self.h, float(dtk)))
+
+    def drift(self, dtd):
+        self._chk(self.L.pn_drift(self.h, float(dtd)))
+
+    def evolve(self, dtk, dtd, sync=True):
+        dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+        dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+        if dtk.size != dtd.size:
+            raise ValueError("dtk and dtd differ in length")
+        self._chk(self.L.pn_evolve(self.h, int(dtk.size), _dptr(dtk), _dptr(dtd)))
+        if sync:
+            self.sync()
+
+    def evolve_timed(self, dtk, dtd):
+        """Runs the steps between two HIP events on the context's stream; returns ms."""
+        dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+        dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+        ms = C.c_double()
+        self._chk(self.L.pn_evolve_timed(self.h, int(dtk.size), _dptr(dtk), _dptr(dtd), C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._chk(self.L.pn_sync(self.h))
+
+    # analysis -------------------------------------------------------------
+    def volume_averages(self):
+        out = PnAverages()
+        self._chk(self.L.pn_volume_averages(self.h, C.byref(out)))
+        return out
+
+    def extrema(self):
+        out = PnExtrema()
+        self._chk(self.L.pn_extrema_get(self.h, C.byref(out)))
+        return out
+
+    def spectrum(self):
+        nb = self.L.pn_spectrum_nbins(self.N, self.a)
+        counts = np.zeros(nb, dtype=np.int32)
+        f = np.zeros(nb)
+        k = np.zeros(nb)
+        q = np.zeros(nb)
+        s = PnSpectrum(nb, 0.0, counts.ctypes.data_as(C.POINTER(C.c_int)), _dptr(f), _dptr(k), _dptr(q))
+        self._chk(self.L.pn_spectrum_get(self.h, C.byref(s)))
+        return {"nbins": s.nbins, "binsize": s.binsize, "counts": counts, "F": f, "K": k, "Q": q}
+
+    def fft_modes(self, field):
+        H = self.N // 2 + 1
+        out = np.empty(2 * self.N * self.N * H)
+        self._chk(self.L.pn_fft_modes(self.h, int(field), _dptr(out)))
+        return out.view(np.complex128).reshape(self.N, self.N, H)
+
+    def islands(self, mode, thr, capacity=65536, labels=False):
+        arr = (PnIsland * max(capacity, 1))()
+        lab = np.empty(self.N ** 3, dtype=np.int32) if labels else None
+        cat = PnCatalog(capacity, 0, 0.0, arr,
+                        lab.ctypes.data_as(C.POINTER(C.c_int)) if labels else None)
+        self._chk(self.L.pn_islands(self.h, int(mode), float(thr), C.byref(cat)))
+        rows = [arr[i] for i in range(cat.count)]
+        return rows, lab, cat.threshold
+
+    def energy_density(self):
+        rho = np.empty(self.N ** 3)
+        self._chk(self.L.pn_energy_density(self.h, _dptr(rho)))
+        return rho
+
+
+def time_factors(T0, dtbase, a, nsteps):
+    """Host replay of the leapfrog factors (reset/evolve, dynamics.cpp:28-50,196-237).
+
+    Returns (dtk, dtd, tau_after) with dtk[n] = tau - Last_Momenta_update at the
+    kick of step n and dtd[n] = tau - Last_Field_update at its drift.
+    """
+    tau = float(T0)
+    dtau = float(dtbase) * float(a)
+    last_f = tau
+    last_m = tau
+    dtk = np.empty(nsteps)
+    dtd = np.empty(nsteps)
+    taus = np.empty(nsteps)
+    half = 0.5 * dtau
+    for n in range(nsteps):
+        tau = tau + half
+        dtk[n] = tau - last_m
+        last_m = tau
+        tau = tau + half
+        dtd[n] = tau - last_f
+        last_f = tau
+        taus[n] = tau
+    return dtk, dtd, taus

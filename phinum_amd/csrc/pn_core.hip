@@ -1,0 +1,725 @@
+// pn_core.hip -- context, field transfer, the fused leapfrog stencil step and
+// the volume reductions of PhiNum's lattice hot path on gfx950.
+//
+// Device layout: Phi and Pi are dense structure-of-arrays fields of N^3
+// values, x fastest (the reference's pos() order without halo planes).  Phi
+// is double-buffered: the fused step reads Phi[cur] and writes Phi[cur^1],
+// so a site's drift never races with a neighbour's kick; Pi is updated in
+// place (each site reads and writes only its own Pi).  The periodic wrap that
+// the reference implements with halo planes and Communication::exchange
+// (comm.cpp:63-78) and with mod() (util.cpp:12-18) is index arithmetic here.
+//
+// FP64 arithmetic keeps the reference's expression order and its IEEE
+// divisions (dynamics.cpp:80,84) and is compiled with -ffp-contract=off, so
+// each site's update is bit-identical to a strict build of the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pn_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_thread_err;
+
+void pn_set_thread_error(const char *msg) { g_thread_err = msg ? msg : ""; }
+
+int pn_fail(pn_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_thread_err = buf;
+  return code;
+}
+
+extern "C" const char *pn_last_error(const pn_ctx *c) {
+  return c ? c->err.c_str() : g_thread_err.c_str();
+}
+
+extern "C" int pn_abi_version(void) { return PN_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// launch geometry
+// ---------------------------------------------------------------------------
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X.  The remap
+// gives XCD k a contiguous run of logical tiles, so tiles that share y and z
+// halo rows are processed on the same XCD and find each other's planes in
+// that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int q = nblocks >> 3, r = nblocks & 7;
+  const int k = b & 7, j = b >> 3;
+  return k * q + (k < r ? k : r) + j;
+}
+
+struct StepGeom {
+  int N, XT, YT, ZT;
+};
+
+static inline StepGeom step_geom(int N, int by, int zb) {
+  StepGeom g;
+  g.N = N;
+  g.XT = (N + 63) / 64;
+  g.YT = (N + by - 1) / by;
+  g.ZT = (N + zb - 1) / zb;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// K1: fused kick + drift (updateMomenta, updateFields, exchange)
+// ---------------------------------------------------------------------------
+struct CoefF64 {
+  double a2, inv_a2, c2, lmode, kappa;
+};
+struct CoefF32 {
+  float inv_a2, c2, c4, c6;
+};
+
+// Theory-0 update of one site, reference order (dynamics.cpp:80,84,183).
+template <bool POW2>
+__device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, double xm, double yp,
+                                                double ym, double zp, double zm, double f,
+                                                double &p, double dtk) {
+  double s = xp + xm;
+  s = s + yp;
+  s = s + ym;
+  s = s + zp;
+  s = s + zm;
+  double lap = POW2 ? (s - 6. * f) * k.inv_a2 : (s - 6. * f) / k.a2;
+  double q = 0.0 + f * f;
+  double B = k.lmode * (q / 6.);
+  double t = q / k.kappa;
+  double C = (t * t) / 120.;
+  p = p + (lap - ((k.c2 + B) + C) * f) * dtk;
+}
+
+// FP32 form: same stencil, potential with host-rounded coefficients.
+__device__ __forceinline__ void site_update_f32(const CoefF32 &k, float xp, float xm, float yp,
+                                                float ym, float zp, float zm, float f, float &p,
+                                                float dtk) {
+  float s = xp + xm;
+  s = s + yp;
+  s = s + ym;
+  s = s + zp;
+  s = s + zm;
+  float lap = (s - 6.0f * f) * k.inv_a2;
+  float q = f * f;
+  float force = ((k.c2 + k.c4 * q) + k.c6 * (q * q)) * f;
+  p = p + (lap - force) * dtk;
+}
+
+// One thread per (x,y) column of a 64 x BY tile, streaming ZB planes in z and
+// carrying phi(z-1), phi(z), phi(z+1) in registers.  In-plane neighbours are
+// read from the same or adjacent cache lines (x) or from rows the tile's other
+// waves have just brought into L1/L2 (y).  MODE 0: fused step (writes phin);
+// MODE 1: kick only (Pi in place).
+template <typename T, typename C, bool POW2, int BY, int ZB, int MODE>
+__global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *__restrict__ pi,
+                                                  T *__restrict__ phin, StepGeom g, C k, T dtk,
+                                                  T dtd) {
+  const int nblocks = g.XT * g.YT * g.ZT;
+  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int xt = t % g.XT;
+  const int yt = (t / g.XT) % g.YT;
+  const int zt = t / (g.XT * g.YT);
+  const int N = g.N;
+  const int x = xt * 64 + (int)threadIdx.x;
+  const int y = yt * BY + (int)threadIdx.y;
+  if (x >= N || y >= N) return;
+  const int z0 = zt * ZB;
+  const int z1 = min(z0 + ZB, N);
+  const int xp = (x + 1 == N) ? 0 : x + 1;
+  const int xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1;
+  const int ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  const int zm0 = (z0 == 0) ? N - 1 : z0 - 1;
+  T fm = phi[zm0 * P + row + x];
+  T fc = phi[z0 * P + row + x];
+  for (int z = z0; z < z1; z++) {
+    const int zp = (z + 1 == N) ? 0 : z + 1;
+    const long long base = z * P;
+    const T fp = phi[zp * P + row + x];
+    const T vxp = phi[base + row + xp];
+    const T vxm = phi[base + row + xm];
+    const T vyp = phi[base + rowp + x];
+    const T vym = phi[base + rowm + x];
+    T p = __builtin_nontemporal_load(&pi[base + row + x]);
+    if constexpr (sizeof(T) == 8) {
+      site_update_f64<POW2>(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
+    } else {
+      site_update_f32(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
+    }
+    __builtin_nontemporal_store(p, &pi[base + row + x]);
+    if constexpr (MODE == 0) {
+      const T fnew = fc + p * dtd;
+      __builtin_nontemporal_store(fnew, &phin[base + row + x]);
+    }
+    fm = fc;
+    fc = fp;
+  }
+}
+
+// updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.
+template <typename T>
+__global__ void k_drift(T *__restrict__ phi, const T *__restrict__ pi, long long n, T dtd) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) phi[i] = phi[i] + pi[i] * dtd;
+}
+
+// tile shape used by the single-lattice path (the batched path autotunes)
+static constexpr int kBY = 4;
+static constexpr int kZB = 16;
+
+template <typename T, typename C, bool POW2, int MODE>
+static void launch_step_t(pn_ctx *c, const C &k, T dtk, T dtd) {
+  StepGeom g = step_geom(c->p.N, kBY, kZB);
+  int nblocks = g.XT * g.YT * g.ZT;
+  const T *phi = (const T *)c->phi[c->cur];
+  T *phin = (T *)c->phi[c->cur ^ 1];
+  hipLaunchKernelGGL((k_step<T, C, POW2, kBY, kZB, MODE>), dim3(nblocks), dim3(64, kBY), 0,
+                     c->stream, phi, (T *)c->pi, phin, g, k, dtk, dtd);
+}
+
+static CoefF64 coef64(const pn_ctx *c) {
+  CoefF64 k;
+  k.a2 = c->k.a2;
+  k.inv_a2 = c->k.inv_a2;
+  k.c2 = c->k.c2;
+  k.lmode = c->k.lmode;
+  k.kappa = c->k.kappa;
+  return k;
+}
+
+static CoefF32 coef32(const pn_ctx *c) {
+  CoefF32 k;
+  k.inv_a2 = (float)(1.0 / c->k.a2);
+  k.c2 = (float)c->k.c2;
+  k.c4 = (float)(c->k.lmode / 6.0);
+  k.c6 = (float)(1.0 / (120.0 * c->k.kappa * c->k.kappa));
+  return k;
+}
+
+// MODE 0 fused step, MODE 1 kick only
+static int launch_step(pn_ctx *c, int mode, double dtk, double dtd) {
+  if (c->p.precision == PN_FP64) {
+    CoefF64 k = coef64(c);
+    if (c->k.a2_pow2) {
+      if (mode == 0) launch_step_t<double, CoefF64, true, 0>(c, k, dtk, dtd);
+      else launch_step_t<double, CoefF64, true, 1>(c, k, dtk, dtd);
+    } else {
+      if (mode == 0) launch_step_t<double, CoefF64, false, 0>(c, k, dtk, dtd);
+      else launch_step_t<double, CoefF64, false, 1>(c, k, dtk, dtd);
+    }
+  } else {
+    CoefF32 k = coef32(c);
+    if (mode == 0) launch_step_t<float, CoefF32, true, 0>(c, k, (float)dtk, (float)dtd);
+    else launch_step_t<float, CoefF32, true, 1>(c, k, (float)dtk, (float)dtd);
+  }
+  PN_CHECK_LAUNCH(c);
+  if (mode == 0) c->cur ^= 1;
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static bool is_pow2_double(double v) {
+  if (!(v > 0) || !std::isfinite(v)) return false;
+  int e;
+  double m = std::frexp(v, &e);
+  return m == 0.5 && e > -1000 && e < 1000;
+}
+
+int pn_reduce_blocks(const pn_ctx *c) { return c->red_blocks; }
+
+static constexpr int kRedThreads = 256;
+static constexpr int kNSums = 9;   // phi, pi, phi2, pi2, phipi, phi4, phi6, grad, inter
+
+extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
+  if (!out) return pn_fail(nullptr, PN_ERR_ARG, "pn_create: out is NULL");
+  *out = nullptr;
+  if (!p) return pn_fail(nullptr, PN_ERR_ARG, "pn_create: params is NULL");
+  if (p->N < 4 || p->N > 2048)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_create: N=%d outside [4, 2048]", p->N);
+  if (!(p->a > 0) || !std::isfinite(p->a))
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_create: lattice spacing a must be > 0");
+  if (p->precision != PN_FP64 && p->precision != PN_FP32)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_create: unknown precision %d", p->precision);
+  if (!(p->kappa != 0) || !std::isfinite(p->kappa))
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_create: Kappa must be finite and nonzero");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return pn_fail(nullptr, PN_ERR_HIP, "pn_create: no HIP device (%s)", hipGetErrorString(e));
+  if (p->device < 0 || p->device >= ndev)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_create: device %d of %d", p->device, ndev);
+
+  pn_ctx *c = new (std::nothrow) pn_ctx();
+  if (!c) return pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: out of host memory");
+  c->p = *p;
+  c->n3 = (long long)p->N * p->N * p->N;
+  c->esize = p->precision == PN_FP64 ? 8 : 4;
+  c->k.a2 = p->a * p->a;
+  c->k.a2_pow2 = is_pow2_double(c->k.a2);
+  c->k.inv_a2 = 1.0 / c->k.a2;
+  c->k.c2 = p->mass_mode * p->m2;
+  c->k.lmode = (double)p->lambda_mode;
+  c->k.kappa = p->kappa;
+  c->red_blocks = (int)std::min<long long>(1024, (c->n3 + kRedThreads - 1) / kRedThreads);
+
+  auto cleanup = [&](int code) {
+    pn_destroy(c);
+    return code;
+  };
+  if (hipSetDevice(p->device) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: stream creation failed"));
+  size_t bytes = (size_t)c->n3 * c->esize;
+  for (int i = 0; i < 2; i++)
+    if (hipMalloc(&c->phi[i], bytes) != hipSuccess)
+      return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate %zu bytes", bytes));
+  if (hipMalloc(&c->pi, bytes) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate %zu bytes", bytes));
+  if (hipMalloc(&c->red_part, (size_t)c->red_blocks * 2 * kNSums * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->ext_part, (size_t)c->red_blocks * 6 * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->red_out, 64 * sizeof(double)) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate reduction scratch"));
+  if (hipHostMalloc(&c->host_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate pinned staging"));
+  if (hipMemsetAsync(c->phi[0], 0, bytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->phi[1], 0, bytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->pi, 0, bytes, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: zero fill failed"));
+  c->cur = 0;
+  *out = c;
+  return PN_OK;
+}
+
+extern "C" int pn_destroy(pn_ctx *c) {
+  if (!c) return PN_OK;
+  hipSetDevice(c->p.device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  pn_free_spectrum(c);
+  pn_free_islands(c);
+  for (int i = 0; i < 2; i++)
+    if (c->phi[i]) hipFree(c->phi[i]);
+  if (c->pi) hipFree(c->pi);
+  if (c->red_part) hipFree(c->red_part);
+  if (c->ext_part) hipFree(c->ext_part);
+  if (c->red_out) hipFree(c->red_out);
+  if (c->host_pin) hipHostFree(c->host_pin);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+static int put_field(pn_ctx *c, void *dst, const double *src, int layout) {
+  const double *interior = layout == PN_LAYOUT_REF ? src + (long long)c->p.N * c->p.N : src;
+  size_t n = (size_t)c->n3;
+  if (c->p.precision == PN_FP64) {
+    PN_HIP(c, hipMemcpyAsync(dst, interior, n * 8, hipMemcpyHostToDevice, c->stream));
+    PN_HIP(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; i++) tmp[i] = (float)interior[i];
+    PN_HIP(c, hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    PN_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return PN_OK;
+}
+
+static int get_field(pn_ctx *c, const void *src, double *dst, int layout) {
+  const long long plane = (long long)c->p.N * c->p.N;
+  double *interior = layout == PN_LAYOUT_REF ? dst + plane : dst;
+  size_t n = (size_t)c->n3;
+  if (c->p.precision == PN_FP64) {
+    PN_HIP(c, hipMemcpyAsync(interior, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+    PN_HIP(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    PN_HIP(c, hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+    PN_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) interior[i] = (double)tmp[i];
+  }
+  if (layout == PN_LAYOUT_REF) {
+    const int N = c->p.N;
+    std::memcpy(dst + plane * (N + 1), dst + plane, plane * sizeof(double));
+    std::memcpy(dst, dst + plane * N, plane * sizeof(double));
+  }
+  return PN_OK;
+}
+
+extern "C" int pn_set_fields(pn_ctx *c, const double *phi, const double *pi, int layout) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_set_fields: NULL context");
+  if (layout != PN_LAYOUT_REF && layout != PN_LAYOUT_DENSE)
+    return pn_fail(c, PN_ERR_ARG, "pn_set_fields: unknown layout %d", layout);
+  PN_HIP(c, hipSetDevice(c->p.device));
+  int rc;
+  if (phi && (rc = put_field(c, c->phi[c->cur], phi, layout)) != PN_OK) return rc;
+  if (pi && (rc = put_field(c, c->pi, pi, layout)) != PN_OK) return rc;
+  return PN_OK;
+}
+
+extern "C" int pn_get_fields(pn_ctx *c, double *phi, double *pi, int layout) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_get_fields: NULL context");
+  if (layout != PN_LAYOUT_REF && layout != PN_LAYOUT_DENSE)
+    return pn_fail(c, PN_ERR_ARG, "pn_get_fields: unknown layout %d", layout);
+  PN_HIP(c, hipSetDevice(c->p.device));
+  int rc;
+  if (phi && (rc = get_field(c, c->phi[c->cur], phi, layout)) != PN_OK) return rc;
+  if (pi && (rc = get_field(c, c->pi, pi, layout)) != PN_OK) return rc;
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dynamics entry points
+// ---------------------------------------------------------------------------
+extern "C" int pn_kick(pn_ctx *c, double dtk) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_kick: NULL context");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  int rc = launch_step(c, 1, dtk, 0.0);
+  if (rc != PN_OK) return rc;
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+extern "C" int pn_drift(pn_ctx *c, double dtd) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_drift: NULL context");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  const int threads = 256;
+  const int blocks = (int)std::min<long long>(4096, (c->n3 + threads - 1) / threads);
+  if (c->p.precision == PN_FP64)
+    hipLaunchKernelGGL(k_drift<double>, dim3(blocks), dim3(threads), 0, c->stream,
+                       (double *)c->phi[c->cur], (const double *)c->pi, c->n3, dtd);
+  else
+    hipLaunchKernelGGL(k_drift<float>, dim3(blocks), dim3(threads), 0, c->stream,
+                       (float *)c->phi[c->cur], (const float *)c->pi, c->n3, (float)dtd);
+  PN_CHECK_LAUNCH(c);
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double *dtd) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_evolve: NULL context");
+  if (nsteps < 0 || (nsteps > 0 && (!dtk || !dtd)))
+    return pn_fail(c, PN_ERR_ARG, "pn_evolve: bad step arguments");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  for (int n = 0; n < nsteps; n++) {
+    int rc = launch_step(c, 0, dtk[n], dtd[n]);
+    if (rc != PN_OK) return rc;
+  }
+  return PN_OK;
+}
+
+extern "C" int pn_evolve_timed(pn_ctx *c, int nsteps, const double *dtk, const double *dtd,
+                               double *ms) {
+  if (!c || !ms) return pn_fail(c, PN_ERR_ARG, "pn_evolve_timed: NULL argument");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  hipEvent_t e0, e1;
+  PN_HIP(c, hipEventCreate(&e0));
+  PN_HIP(c, hipEventCreate(&e1));
+  int rc = PN_OK;
+  hipError_t he = hipEventRecord(e0, c->stream);
+  if (he == hipSuccess) rc = pn_evolve(c, nsteps, dtk, dtd);
+  if (he == hipSuccess && rc == PN_OK) he = hipEventRecord(e1, c->stream);
+  if (he == hipSuccess && rc == PN_OK) he = hipEventSynchronize(e1);
+  float f = 0.f;
+  if (he == hipSuccess && rc == PN_OK) he = hipEventElapsedTime(&f, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PN_OK) return rc;
+  if (he != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_evolve_timed: %s", hipGetErrorString(he));
+  *ms = f;
+  return PN_OK;
+}
+
+extern "C" int pn_sync(pn_ctx *c) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_sync: NULL context");
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// K3/K4: volume sums and extrema (fieldanalysis.cpp:319-699)
+// ---------------------------------------------------------------------------
+// Sums are FP64 and compensated: each thread keeps (hi, lo) pairs updated
+// with an exact two-sum, partial pairs are merged in a fixed tree, so the
+// result is deterministic and within a few ulp of the exact sum.
+__device__ __forceinline__ void two_acc(double &hi, double &lo, double x) {
+  const double s = hi + x;
+  const double bp = s - hi;
+  const double e = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo = lo + e;
+}
+
+__device__ __forceinline__ void two_merge(double &h1, double &l1, double h2, double l2) {
+  two_acc(h1, l1, h2);
+  l1 = l1 + l2;
+}
+
+struct RedCoef {
+  double a2;    // sqr(a_t)
+  double cm;    // (Mass_Mode*0.5)*BareM_sqr_long        (fieldanalysis.cpp:457)
+  double lmode; // Lambda_Mode                           (:571)
+  double d6;    // 720.0*Nc*Nc*Kappa*Kappa               (:571)
+};
+
+// extrema candidates: strict comparisons in raster order, ties -> lowest index
+struct Ext {
+  double mn, mx, ab;
+  long long imn, imx, iab;
+};
+
+__device__ __forceinline__ void ext_merge(Ext &a, const Ext &b) {
+  if (b.mn < a.mn || (b.mn == a.mn && b.imn >= 0 && (a.imn < 0 || b.imn < a.imn))) {
+    a.mn = b.mn; a.imn = b.imn;
+  }
+  if (b.mx > a.mx || (b.mx == a.mx && b.imx >= 0 && (a.imx < 0 || b.imx < a.imx))) {
+    a.mx = b.mx; a.imx = b.imx;
+  }
+  if (b.ab > a.ab || (b.ab == a.ab && b.iab >= 0 && (a.iab < 0 || b.iab < a.iab))) {
+    a.ab = b.ab; a.iab = b.iab;
+  }
+}
+
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ long long shfl_ll(long long v, int src) { return __shfl(v, src, 64); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_volume(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                int N, RedCoef k, double *__restrict__ part,
+                                                long long *__restrict__ epart) {
+  double hi[kNSums], lo[kNSums];
+#pragma unroll
+  for (int s = 0; s < kNSums; s++) { hi[s] = 0.0; lo[s] = 0.0; }
+  Ext e;
+  e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;  // fieldanalysis.cpp:324
+  e.imn = e.imx = e.iab = -1;
+  const long long n3 = (long long)N * N * N;
+  const long long P = (long long)N * N;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+    const int x = (int)(i % N);
+    const int y = (int)((i / N) % N);
+    const int z = (int)(i / P);
+    const long long ixp = i - x + (x + 1 == N ? 0 : x + 1);
+    const long long iyp = i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N;
+    const long long izp = i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P;
+    const double f = (double)phi[i], p = (double)pi[i];
+    const double fx = (double)phi[ixp], fy = (double)phi[iyp], fz = (double)phi[izp];
+    double q = 0.0;
+    q += f * f;
+    two_acc(hi[0], lo[0], f);
+    two_acc(hi[1], lo[1], p);
+    two_acc(hi[2], lo[2], f * f);
+    two_acc(hi[3], lo[3], p * p);
+    two_acc(hi[4], lo[4], f * p);
+    two_acc(hi[5], lo[5], q * q);
+    two_acc(hi[6], lo[6], (q * q) * q);
+    const double dx = fx - f, dy = fy - f, dz = fz - f;
+    two_acc(hi[7], lo[7], (dx * dx / k.a2 + dy * dy / k.a2) + dz * dz / k.a2);  // :445
+    two_acc(hi[8], lo[8], k.cm * (f * f));                                        // :457
+    two_acc(hi[8], lo[8], k.lmode * ((q * q) / 24.0) + ((q * q) * q) / k.d6);     // :571
+    // extrema (fieldanalysis.cpp:331-355); AbsPhi = sqrt(0 + phi*phi)
+    const long long rp = i + P;
+    const double ab = sqrt(0.0 + f * f);
+    if (ab > e.ab) { e.ab = ab; e.iab = rp; }
+    if (f > e.mx) { e.mx = f; e.imx = rp; }
+    if (f < e.mn) { e.mn = f; e.imn = rp; }
+  }
+  // wavefront reduction
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int s = 0; s < kNSums; s++) {
+      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
+      two_merge(hi[s], lo[s], h2, l2);
+    }
+    Ext o;
+    o.mn = __shfl_down(e.mn, off, 64); o.imn = __shfl_down(e.imn, off, 64);
+    o.mx = __shfl_down(e.mx, off, 64); o.imx = __shfl_down(e.imx, off, 64);
+    o.ab = __shfl_down(e.ab, off, 64); o.iab = __shfl_down(e.iab, off, 64);
+    ext_merge(e, o);
+  }
+  __shared__ double sh[4][2 * kNSums];
+  __shared__ Ext se[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < kNSums; s++) { sh[w][2 * s] = hi[s]; sh[w][2 * s + 1] = lo[s]; }
+    se[w] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ww = 1; ww < (int)(blockDim.x >> 6); ww++) {
+      for (int s = 0; s < kNSums; s++) two_merge(sh[0][2 * s], sh[0][2 * s + 1], sh[ww][2 * s], sh[ww][2 * s + 1]);
+      ext_merge(se[0], se[ww]);
+    }
+    for (int s = 0; s < 2 * kNSums; s++) part[(long long)blockIdx.x * 2 * kNSums + s] = sh[0][s];
+    long long *ep = epart + (long long)blockIdx.x * 6;
+    ep[0] = __double_as_longlong(se[0].mn); ep[1] = se[0].imn;
+    ep[2] = __double_as_longlong(se[0].mx); ep[3] = se[0].imx;
+    ep[4] = __double_as_longlong(se[0].ab); ep[5] = se[0].iab;
+  }
+}
+
+// single wave merges the block partials in a fixed order
+__global__ __launch_bounds__(64) void k_volume_final(const double *__restrict__ part,
+                                                     const long long *__restrict__ epart, int nb,
+                                                     double *__restrict__ out) {
+  double hi[kNSums], lo[kNSums];
+#pragma unroll
+  for (int s = 0; s < kNSums; s++) { hi[s] = 0.0; lo[s] = 0.0; }
+  Ext e;
+  e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;
+  e.imn = e.imx = e.iab = -1;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    for (int s = 0; s < kNSums; s++)
+      two_merge(hi[s], lo[s], part[(long long)b * 2 * kNSums + 2 * s], part[(long long)b * 2 * kNSums + 2 * s + 1]);
+    const long long *ep = epart + (long long)b * 6;
+    Ext o;
+    o.mn = __longlong_as_double(ep[0]); o.imn = ep[1];
+    o.mx = __longlong_as_double(ep[2]); o.imx = ep[3];
+    o.ab = __longlong_as_double(ep[4]); o.iab = ep[5];
+    ext_merge(e, o);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int s = 0; s < kNSums; s++) {
+      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
+      two_merge(hi[s], lo[s], h2, l2);
+    }
+    Ext o;
+    o.mn = __shfl_down(e.mn, off, 64); o.imn = __shfl_down(e.imn, off, 64);
+    o.mx = __shfl_down(e.mx, off, 64); o.imx = __shfl_down(e.imx, off, 64);
+    o.ab = __shfl_down(e.ab, off, 64); o.iab = __shfl_down(e.iab, off, 64);
+    ext_merge(e, o);
+  }
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < kNSums; s++) out[s] = hi[s] + lo[s];
+    out[kNSums + 0] = e.mn; out[kNSums + 1] = (double)e.imn;
+    out[kNSums + 2] = e.mx; out[kNSums + 3] = (double)e.imx;
+    out[kNSums + 4] = e.ab; out[kNSums + 5] = (double)e.iab;
+  }
+}
+
+static RedCoef red_coef(const pn_ctx *c) {
+  RedCoef k;
+  const int Nc = 1;
+  k.a2 = c->p.a * c->p.a;
+  k.cm = (c->p.mass_mode) * 0.5 * c->p.m2;
+  k.lmode = (double)c->p.lambda_mode;
+  k.d6 = 720.0 * Nc * Nc * c->p.kappa * c->p.kappa;
+  return k;
+}
+
+// runs K3/K4 and leaves 9 sums + 6 extrema values in c->host_pin
+static int run_volume(pn_ctx *c) {
+  PN_HIP(c, hipSetDevice(c->p.device));
+  RedCoef k = red_coef(c);
+  const int nb = c->red_blocks;
+  if (c->p.precision == PN_FP64)
+    hipLaunchKernelGGL(k_volume<double>, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                       (const double *)c->phi[c->cur], (const double *)c->pi, c->p.N, k,
+                       c->red_part, c->ext_part);
+  else
+    hipLaunchKernelGGL(k_volume<float>, dim3(nb), dim3(kRedThreads), 0, c->stream,
+                       (const float *)c->phi[c->cur], (const float *)c->pi, c->p.N, k,
+                       c->red_part, c->ext_part);
+  PN_CHECK_LAUNCH(c);
+  hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c->stream, c->red_part, c->ext_part,
+                     nb, c->red_out);
+  PN_CHECK_LAUNCH(c);
+  PN_HIP(c, hipMemcpyAsync(c->host_pin, c->red_out, (kNSums + 6) * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+extern "C" int pn_volume_averages(pn_ctx *c, pn_averages *out) {
+  if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_volume_averages: NULL argument");
+  int rc = run_volume(c);
+  if (rc != PN_OK) return rc;
+  const double *s = c->host_pin;
+  const int vol = c->p.N * c->p.N * c->p.N;
+  // final arithmetic as fieldanalysis.cpp:619-638 (one rank: /nodes == /1)
+  double phi1 = s[0], pi1 = s[1], phi2 = s[2], pi2 = s[3], phipi = s[4], phi4 = s[5], phi6 = s[6];
+  double G = s[7], I = s[8];
+  phi1 /= (vol); pi1 /= (vol);
+  double Charge = (0.5 * pi2 - G / 6. - I);
+  double Energy = (0.5 * pi2 + 0.5 * G + I);
+  Energy /= (vol); I /= (vol); Charge /= (vol);
+  phi2 /= (vol); pi2 /= (vol); phi4 /= (vol); phi6 /= (vol); phipi /= (vol);
+  out->phi = phi1; out->pi = pi1; out->phi2 = phi2; out->pi2 = pi2; out->phipi = phipi;
+  out->phi4 = phi4; out->phi6 = phi6; out->energy = Energy; out->interaction = I;
+  out->charge = Charge;
+  return PN_OK;
+}
+
+extern "C" int pn_extrema_get(pn_ctx *c, pn_extrema *out) {
+  if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_extrema_get: NULL argument");
+  int rc = run_volume(c);
+  if (rc != PN_OK) return rc;
+  const double *s = c->host_pin + kNSums;
+  out->min = s[0]; out->minpos = (long long)s[1];
+  out->max = s[2]; out->maxpos = (long long)s[3];
+  out->maxabs = s[4]; out->maxabspos = (long long)s[5];
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-site energy density (the summand of Energy, fieldanalysis.cpp:625)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_rho(const T *__restrict__ phi, const T *__restrict__ pi, int N, RedCoef k,
+                      double *__restrict__ rho) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    const double f = (double)phi[i], p = (double)pi[i];
+    const double dx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)] - f;
+    const double dy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N] - f;
+    const double dz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P] - f;
+    const double q = 0.0 + f * f;
+    const double g = (dx * dx / k.a2 + dy * dy / k.a2) + dz * dz / k.a2;
+    const double v = k.cm * (f * f) + (k.lmode * ((q * q) / 24.0) + ((q * q) * q) / k.d6);
+    rho[i] = (0.5 * (p * p) + 0.5 * g) + v;
+  }
+}
+
+extern "C" int pn_energy_density(pn_ctx *c, double *rho) {
+  if (!c || !rho) return pn_fail(c, PN_ERR_ARG, "pn_energy_density: NULL argument");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  double *d = nullptr;
+  PN_HIP(c, hipMalloc(&d, (size_t)c->n3 * sizeof(double)));
+  RedCoef k = red_coef(c);
+  const int blocks = (int)std::min<long long>(8192, (c->n3 + 255) / 256);
+  if (c->p.precision == PN_FP64)
+    hipLaunchKernelGGL(k_rho<double>, dim3(blocks), dim3(256), 0, c->stream,
+                       (const double *)c->phi[c->cur], (const double *)c->pi, c->p.N, k, d);
+  else
+    hipLaunchKernelGGL(k_rho<float>, dim3(blocks), dim3(256), 0, c->stream,
+                       (const float *)c->phi[c->cur], (const float *)c->pi, c->p.N, k, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(rho, d, (size_t)c->n3 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_energy_density: %s", hipGetErrorString(e));
+  return PN_OK;
+}

@@ -1,0 +1,68 @@
+// pn_internal.h -- context layout and helpers shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/phinum_gpu.h"
+
+struct pn_fft_state;     // pn_spectrum.hip
+struct pn_island_state;  // pn_islands.hip
+
+// Coefficients of the theory-0 force, computed once on the host in the
+// reference's own expression order (dynamics.cpp:84):
+//   force = ((c2 + lmode*(q/6.0)) + ((q/kappa)*(q/kappa))/120.0) * phi
+struct pn_coef {
+  double a2;       // sqr(a_t)
+  double inv_a2;   // 1/a2, used only when a2 is a power of two (exact)
+  int a2_pow2;     // division by a2 may be replaced by the multiplication
+  double c2;       // Mass_Mode * BareM_sqr_long
+  double lmode;    // (double)Lambda_Mode
+  double kappa;
+};
+
+struct pn_ctx {
+  pn_params p;
+  pn_coef k;
+  long long n3;        // N^3 sites
+  size_t esize;        // 8 (FP64) or 4 (FP32)
+  void *phi[2];        // ping-pong field buffers, dense x-fastest
+  void *pi;            // momenta, updated in place
+  int cur;             // phi[cur] holds the current field
+  hipStream_t stream;
+  // reduction scratch
+  double *red_part;    // per-block partial sums
+  double *red_out;     // final sums (device)
+  long long *ext_part; // per-block extrema
+  double *host_pin;    // pinned host staging for small results
+  int red_blocks;
+  pn_fft_state *fft;
+  pn_island_state *isl;
+  std::string err;
+};
+
+// error helpers ------------------------------------------------------------
+int pn_fail(pn_ctx *c, int code, const char *fmt, ...);
+void pn_set_thread_error(const char *msg);
+
+#define PN_HIP(c, call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return pn_fail((c), PN_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                     __FILE__, __LINE__);                                               \
+  } while (0)
+
+#define PN_CHECK_LAUNCH(c) PN_HIP(c, hipGetLastError())
+
+static inline long long pn_ref_pos(const pn_ctx *c, long long dense) {
+  // dense interior index -> reference pos() index (one z halo plane below)
+  return dense + (long long)c->p.N * c->p.N;
+}
+
+// shared by several translation units
+int pn_reduce_blocks(const pn_ctx *c);
+void pn_free_spectrum(pn_ctx *c);
+void pn_free_islands(pn_ctx *c);

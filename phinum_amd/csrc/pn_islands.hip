@@ -1,0 +1,488 @@
+// pn_islands.hip -- island catalog as GPU connected-component labelling.
+//
+// Replaces the queue-based flood fill of the reference (BFS, islands.cpp:90-279,
+// driven by the raster scan of Full_Analysis, :827-835) with a lock-free
+// union-find over the thresholded lattice:
+//   1. mask + init : label[i] = i for island sites, -1 otherwise (is_island,
+//                    islands.cpp:72-87)
+//   2. merge       : every island site unites with its x-1, y-1 (periodic) and
+//                    z-1 (open: the reference never links plane 1 with plane N
+//                    at one rank, islands.cpp:224-236) neighbours with atomicMin
+//                    on roots, so every root ends up being the smallest index
+//                    of its component
+//   3. flatten     : label[i] = root
+//   4. numbering   : roots in increasing index order get 0,1,2,... -- exactly
+//                    the order in which the raster scan meets the islands
+//   5. statistics  : integer counts and offset sums exactly, FP64 energy sums,
+//                    the peak as the largest key (phi^2 or Delta) with ties
+//                    broken towards the smallest index.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pn_internal.h"
+
+struct pn_island_state {
+  int *label = nullptr;      // N^3
+  int *cid = nullptr;        // N^3: island number stored at root positions
+  int *blk = nullptr;        // per-block root counts / offsets
+  long long nblk = 0;
+  int cap = 0;               // allocated catalog entries
+  unsigned long long *ikey = nullptr;  // per island: max key (ordered bits)
+  long long *iint = nullptr;           // per island: nn, first, peak, ties, sdx, sdy, sdz
+  double *idbl = nullptr;              // per island: sum phi, pot, kin, grad, peak value
+  int *count = nullptr;
+};
+
+void pn_free_islands(pn_ctx *c) {
+  pn_island_state *s = c->isl;
+  if (!s) return;
+  hipFree(s->label);
+  hipFree(s->cid);
+  hipFree(s->blk);
+  hipFree(s->ikey);
+  hipFree(s->iint);
+  hipFree(s->idbl);
+  hipFree(s->count);
+  delete s;
+  c->isl = nullptr;
+}
+
+static constexpr int kT = 256;          // threads per block
+static constexpr int kChunk = kT * 16;  // sites per block in the numbering pass
+static constexpr int kNI = 7;           // integer stats per island
+static constexpr int kND = 5;           // double stats per island (+ peak value)
+
+struct IslCoef {
+  double a2, cm, lmode, d6, thr, thr2h;
+  int mode;
+};
+
+// site energy density Delta (fieldanalysis.cpp:447,514-515)
+template <typename T>
+__device__ __forceinline__ double site_delta(const T *phi, const T *pi, long long i, int x, int y,
+                                             int z, int N, const IslCoef &k) {
+  const long long P = (long long)N * N;
+  const double f = (double)phi[i], p = (double)pi[i];
+  const double dx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)] - f;
+  const double dy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N] - f;
+  const double dz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P] - f;
+  double d = (0.5 * (p * p) + (dx * dx) / k.a2 + (dy * dy) / k.a2 + (dz * dz) / k.a2);
+  d += k.cm * (f * f);
+  const double q2 = (f * f) * (f * f);
+  d += k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
+  return d;
+}
+
+template <typename T>
+__device__ __forceinline__ bool site_mask(const T *phi, const T *pi, long long i, int x, int y,
+                                          int z, int N, const IslCoef &k, double *key) {
+  if (k.mode == 0) {
+    const double f = (double)phi[i], p = (double)pi[i];
+    if (key) *key = f * f;
+    return (f * f * 0.5 + p * p * 0.5 > k.thr2h);  // islands.cpp:76
+  }
+  const double d = site_delta(phi, pi, i, x, y, z, N, k);
+  if (key) *key = d;
+  return d > k.thr;  // islands.cpp:83
+}
+
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_init(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                 int N, IslCoef k, int *__restrict__ label) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    label[i] = site_mask(phi, pi, i, x, y, z, N, k, nullptr) ? (int)i : -1;
+  }
+}
+
+// parent reads bypass the non-coherent per-CU cache so that roots written
+// by other CUs' atomics are seen; parents only ever decrease, so any value
+// read is an ancestor and the walk terminates
+__device__ __forceinline__ int uf_load(const int *label, int x) {
+  return __hip_atomic_load(const_cast<int *>(label) + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int uf_find(const int *label, int x) {
+  int p = uf_load(label, x);
+  while (p != x) {
+    x = p;
+    p = uf_load(label, x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ void uf_union(int *label, int a, int b) {
+  bool done;
+  do {
+    a = uf_find(label, a);
+    b = uf_find(label, b);
+    if (a < b) {
+      const int old = atomicMin(&label[b], a);
+      done = (old == b);
+      b = old;
+    } else if (b < a) {
+      const int old = atomicMin(&label[a], b);
+      done = (old == a);
+      a = old;
+    } else {
+      done = true;
+    }
+  } while (!done);
+}
+
+__global__ __launch_bounds__(kT) void k_isl_merge(int *label, int N) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    if (label[i] < 0) continue;
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    const long long jx = i - x + (x == 0 ? N - 1 : x - 1);
+    const long long jy = i + (long long)((y == 0 ? N - 1 : y - 1) - y) * N;
+    if (label[jx] >= 0) uf_union(label, (int)i, (int)jx);
+    if (label[jy] >= 0) uf_union(label, (int)i, (int)jy);
+    if (z > 0 && label[i - P] >= 0) uf_union(label, (int)i, (int)(i - P));
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_isl_flatten(int *__restrict__ label, long long n3) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    if (label[i] >= 0) label[i] = uf_find(label, (int)i);
+  }
+}
+
+// numbering: count roots per chunk, scan chunk counts, assign in order
+__global__ __launch_bounds__(kT) void k_isl_count(const int *__restrict__ label, long long n3,
+                                                  int *__restrict__ blk) {
+  __shared__ int s[kT];
+  const long long base = (long long)blockIdx.x * kChunk;
+  int cnt = 0;
+  for (int j = threadIdx.x; j < kChunk; j += kT) {
+    const long long i = base + j;
+    if (i < n3 && label[i] == (int)i) cnt++;
+  }
+  s[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[blockIdx.x] = s[0];
+}
+
+// exclusive scan of nb block counts by one block; total in blk[nb]
+__global__ __launch_bounds__(1024) void k_isl_scan(int *__restrict__ blk, long long nb) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < nb; base += 1024) {
+    const long long i = base + threadIdx.x;
+    const int v = i < nb ? blk[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) blk[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blk[nb] = carry;
+}
+
+__global__ __launch_bounds__(kT) void k_isl_assign(const int *__restrict__ label, long long n3,
+                                                   const int *__restrict__ blk,
+                                                   int *__restrict__ cid) {
+  __shared__ int s[kT];
+  __shared__ int carry;
+  const long long base = (long long)blockIdx.x * kChunk;
+  if (threadIdx.x == 0) carry = blk[blockIdx.x];
+  __syncthreads();
+  for (int j0 = 0; j0 < kChunk; j0 += kT) {
+    const long long i = base + j0 + threadIdx.x;
+    const int r = (i < n3 && label[i] == (int)i) ? 1 : 0;
+    s[threadIdx.x] = r;
+    __syncthreads();
+    for (int o = 1; o < kT; o <<= 1) {
+      const int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (r) cid[i] = carry + s[threadIdx.x] - 1;
+    __syncthreads();
+    if (threadIdx.x == kT - 1) carry += s[kT - 1];
+    __syncthreads();
+  }
+}
+
+// order-preserving map of a double onto unsigned 64-bit integers
+__device__ __forceinline__ unsigned long long ord_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+struct IslStatCoef {
+  double a2, cm, lmode, d6;
+};
+
+// pass A: sizes, first site, sums, max key
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_stats(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                  int N, IslCoef k, const int *__restrict__ label,
+                                                  const int *__restrict__ cid,
+                                                  unsigned long long *__restrict__ ikey,
+                                                  long long *__restrict__ iint,
+                                                  double *__restrict__ idbl) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    const int r = label[i];
+    if (r < 0) continue;
+    const int id = cid[r];
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    const double f = (double)phi[i], p = (double)pi[i];
+    double key;
+    if (k.mode == 0) key = f * f;
+    else key = site_delta(phi, pi, i, x, y, z, N, k);
+    const double fx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)];
+    const double fy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N];
+    const double fz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P];
+    // islands.cpp:131-142
+    const double grad = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / k.a2;
+    const double kin = 0.5 * (p * p);
+    const double q2 = (f * f) * (f * f);
+    const double pot = k.cm * (f * f) + k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
+    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 0], 1ULL);
+    atomicAdd(&idbl[(long long)id * kND + 0], f);
+    atomicAdd(&idbl[(long long)id * kND + 1], pot);
+    atomicAdd(&idbl[(long long)id * kND + 2], kin);
+    atomicAdd(&idbl[(long long)id * kND + 3], grad);
+    atomicMax(&ikey[id], ord_key(key));
+  }
+}
+
+// pass B: peak = smallest index holding the max key; tie count
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_peak(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                 int N, IslCoef k, const int *__restrict__ label,
+                                                 const int *__restrict__ cid,
+                                                 const unsigned long long *__restrict__ ikey,
+                                                 long long *__restrict__ iint) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    const int r = label[i];
+    if (r < 0) continue;
+    const int id = cid[r];
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    double key;
+    if (k.mode == 0) {
+      const double f = (double)phi[i];
+      key = f * f;
+    } else {
+      key = site_delta(phi, pi, i, x, y, z, N, k);
+    }
+    if (ord_key(key) == ikey[id]) {
+      atomicMin(&iint[(long long)id * kNI + 2], (long long)i);
+      atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 3], 1ULL);
+    }
+  }
+}
+
+// pass C: offsets from the peak (x, y minimum image; z plain)
+__global__ __launch_bounds__(kT) void k_isl_offsets(int N, const int *__restrict__ label,
+                                                    const int *__restrict__ cid,
+                                                    long long *__restrict__ iint,
+                                                    int *__restrict__ out_labels) {
+  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT) {
+    const int r = label[i];
+    if (out_labels) out_labels[i] = r < 0 ? -1 : cid[r];
+    if (r < 0) continue;
+    const int id = cid[r];
+    const long long pk = iint[(long long)id * kNI + 2];
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    const int px = (int)(pk % N), py = (int)((pk / N) % N), pz = (int)(pk / P);
+    int dx = x - px, dy = y - py;
+    if (dx >= N - N / 2) dx -= N;
+    if (dx < -(N / 2)) dx += N;
+    if (dy >= N - N / 2) dy -= N;
+    if (dy < -(N / 2)) dy += N;
+    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 4], (unsigned long long)(long long)dx);
+    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 5], (unsigned long long)(long long)dy);
+    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 6], (unsigned long long)(long long)(z - pz));
+  }
+}
+
+__global__ void k_isl_prep(int n, unsigned long long *__restrict__ ikey, long long *__restrict__ iint,
+                           double *__restrict__ idbl) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  ikey[id] = 0ULL;
+  iint[(long long)id * kNI + 0] = 0;
+  iint[(long long)id * kNI + 1] = 0;
+  iint[(long long)id * kNI + 2] = 0x7fffffffffffffffLL;
+  for (int j = 3; j < kNI; j++) iint[(long long)id * kNI + j] = 0;
+  for (int j = 0; j < kND; j++) idbl[(long long)id * kND + j] = 0.0;
+}
+
+// first site of each island = its root
+__global__ __launch_bounds__(kT) void k_isl_roots(const int *__restrict__ label, long long n3,
+                                                  const int *__restrict__ cid,
+                                                  long long *__restrict__ iint) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
+       i += (long long)gridDim.x * kT)
+    if (label[i] == (int)i) iint[(long long)cid[i] * kNI + 1] = i;
+}
+
+// peak value: Phi (mode 0, islands.cpp:257) or Delta (mode 1, :260) at the peak
+template <typename T>
+__global__ void k_isl_peakval(const T *__restrict__ phi, const T *__restrict__ pi, int N,
+                              IslCoef k, int n, const long long *__restrict__ iint,
+                              double *__restrict__ idbl) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const long long P = (long long)N * N;
+  const long long pk = iint[(long long)id * kNI + 2];
+  const int x = (int)(pk % N), y = (int)((pk / N) % N), z = (int)(pk / P);
+  idbl[(long long)id * kND + 4] =
+      k.mode == 0 ? (double)phi[pk] : site_delta(phi, pi, pk, x, y, z, N, k);
+}
+
+template <typename T>
+static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
+  pn_island_state *s = c->isl;
+  const int N = c->p.N;
+  const long long n3 = c->n3;
+  const int grid = (int)std::min<long long>(8192, (n3 + kT - 1) / kT);
+  const T *phi = (const T *)c->phi[c->cur];
+  const T *pi = (const T *)c->pi;
+  hipLaunchKernelGGL(k_isl_init<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label);
+  hipLaunchKernelGGL(k_isl_merge, dim3(grid), dim3(kT), 0, c->stream, s->label, N);
+  hipLaunchKernelGGL(k_isl_flatten, dim3(grid), dim3(kT), 0, c->stream, s->label, n3);
+  hipLaunchKernelGGL(k_isl_count, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3, s->blk);
+  hipLaunchKernelGGL(k_isl_scan, dim3(1), dim3(1024), 0, c->stream, s->blk, s->nblk);
+  hipLaunchKernelGGL(k_isl_assign, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3,
+                     s->blk, s->cid);
+  PN_CHECK_LAUNCH(c);
+  int nisl = 0;
+  PN_HIP(c, hipMemcpyAsync(&nisl, s->blk + s->nblk, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  out->count = nisl;
+  if (nisl > s->cap) {
+    hipFree(s->ikey);
+    hipFree(s->iint);
+    hipFree(s->idbl);
+    s->ikey = nullptr; s->iint = nullptr; s->idbl = nullptr;
+    s->cap = 0;
+    const int cap = std::max(nisl, 1024);
+    PN_HIP(c, hipMalloc(&s->ikey, (size_t)cap * sizeof(unsigned long long)));
+    PN_HIP(c, hipMalloc(&s->iint, (size_t)cap * kNI * sizeof(long long)));
+    PN_HIP(c, hipMalloc(&s->idbl, (size_t)cap * kND * sizeof(double)));
+    s->cap = cap;
+  }
+  if (nisl > 0) {
+    hipLaunchKernelGGL(k_isl_prep, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, nisl,
+                       s->ikey, s->iint, s->idbl);
+    hipLaunchKernelGGL(k_isl_roots, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, s->iint);
+    hipLaunchKernelGGL(k_isl_stats<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
+                       s->cid, s->ikey, s->iint, s->idbl);
+    hipLaunchKernelGGL(k_isl_peak<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
+                       s->cid, s->ikey, s->iint);
+    hipLaunchKernelGGL(k_isl_peakval<T>, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, phi, pi,
+                       N, k, nisl, s->iint, s->idbl);
+  }
+  int *dlab = nullptr;
+  if (out->labels) PN_HIP(c, hipMalloc(&dlab, (size_t)n3 * sizeof(int)));
+  if (nisl > 0 || dlab)
+    hipLaunchKernelGGL(k_isl_offsets, dim3(grid), dim3(kT), 0, c->stream, N, s->label, s->cid,
+                       s->iint, dlab);
+  hipError_t e = hipGetLastError();
+  std::vector<long long> ii((size_t)nisl * kNI);
+  std::vector<double> dd((size_t)nisl * kND);
+  if (e == hipSuccess && nisl > 0)
+    e = hipMemcpyAsync(ii.data(), s->iint, ii.size() * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && nisl > 0)
+    e = hipMemcpyAsync(dd.data(), s->idbl, dd.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && dlab)
+    e = hipMemcpyAsync(out->labels, dlab, (size_t)n3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (dlab) hipFree(dlab);
+  if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands: %s", hipGetErrorString(e));
+
+  // finalise on the host exactly as BFS does (islands.cpp:240-245)
+  const double a = c->p.a, a3 = (a * a) * a;
+  const long long P = (long long)N * N;
+  const int ncopy = std::min(nisl, out->capacity);
+  for (int id = 0; id < ncopy; id++) {
+    const long long *q = &ii[(size_t)id * kNI];
+    const double *d = &dd[(size_t)id * kND];
+    pn_island &I = out->islands[id];
+    const long long NN = q[0];
+    I.index = id;
+    I.nn = NN;
+    I.first_pos = pn_ref_pos(c, q[1]);
+    I.peak_pos = pn_ref_pos(c, q[2]);
+    I.peak_ties = q[3] - 1;
+    I.volume = NN * a * a * a;
+    I.avg_phi = d[0] / (double)(NN);
+    I.pot_en = d[1] * a3;
+    I.kin_en = d[2] * a3;
+    I.grad_en = d[3] * a3;
+    I.sum_dx = q[4];
+    I.sum_dy = q[5];
+    I.sum_dz = q[6];
+    const long long pk = q[2];
+    const double px = (double)(pk % N), py = (double)((pk / N) % N), pz = (double)(pk / P);
+    double cx = px + (double)q[4] / (double)NN, cy = py + (double)q[5] / (double)NN;
+    cx = std::fmod(cx + N, (double)N);
+    cy = std::fmod(cy + N, (double)N);
+    I.cx = cx;
+    I.cy = cy;
+    I.cz = pz + (double)q[6] / (double)NN;
+    I.peak_value = d[4];
+  }
+  if (nisl > out->capacity)
+    return pn_fail(c, PN_ERR_CAPACITY, "pn_islands: %d islands, capacity %d", nisl, out->capacity);
+  return PN_OK;
+}
+
+extern "C" int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out) {
+  if (!c || !out || (out->capacity > 0 && !out->islands))
+    return pn_fail(c, PN_ERR_ARG, "pn_islands: NULL argument");
+  if (mode != 0 && mode != 1) return pn_fail(c, PN_ERR_ARG, "pn_islands: mode %d", mode);
+  if (c->n3 > 0x7fffffffLL) return pn_fail(c, PN_ERR_ARG, "pn_islands: lattice too large for int labels");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  if (!c->isl) {
+    c->isl = new pn_island_state();
+    pn_island_state *s = c->isl;
+    s->nblk = (c->n3 + kChunk - 1) / kChunk;
+    PN_HIP(c, hipMalloc(&s->label, (size_t)c->n3 * sizeof(int)));
+    PN_HIP(c, hipMalloc(&s->cid, (size_t)c->n3 * sizeof(int)));
+    PN_HIP(c, hipMalloc(&s->blk, (size_t)(s->nblk + 1) * sizeof(int)));
+  }
+  IslCoef k;
+  const int Nc = 1;
+  k.a2 = c->p.a * c->p.a;
+  k.cm = (c->p.mass_mode) * 0.5 * c->p.m2;
+  k.lmode = (double)c->p.lambda_mode;
+  k.d6 = 720.0 * Nc * Nc * c->p.kappa * c->p.kappa;
+  k.thr = thr;
+  k.thr2h = thr * thr * 0.5;
+  k.mode = mode;
+  out->threshold = thr;
+  if (c->p.precision == PN_FP64) return run_islands<double>(c, k, out);
+  return run_islands<float>(c, k, out);
+}

@@ -1,0 +1,190 @@
+"""GPU parity of the fused leapfrog step, reductions, spectrum and islands
+against the oracle (bitwise where the reference's arithmetic is reproduced,
+stated tolerances where the summation order necessarily differs)."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import phinum_amd as pn
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(N, seed, phi0=2.449489742783178, amp=0.4):
+    rng = np.random.default_rng(seed)
+    return phi0 + amp * rng.standard_normal(N ** 3), 0.2 * rng.standard_normal(N ** 3)
+
+
+def _lat(m, precision=pn.PN_FP64):
+    return pn.Lattice(m["N"], m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
+                      precision=precision)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+@pytest.mark.parametrize("N,steps,seed", [(8, 30, 1), (12, 20, 2), (16, 20, 3), (33, 5, 4),
+                                          (64, 10, 5), (100, 3, 6)])
+def test_fused_step_bitwise_fp64(N, steps, seed):
+    m = O.model(N)
+    phi, pi = _fields(N, seed)
+    ophi, opi = O.evolve(m, phi, pi, steps)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        L.evolve(dtk, dtd)
+        gphi, gpi = L.get_fields()
+    assert np.array_equal(_bits(gphi), _bits(ophi))
+    assert np.array_equal(_bits(gpi), _bits(opi))
+
+
+def test_non_pow2_spacing_and_other_couplings_bitwise():
+    m = dict(N=20, a=0.37, mass_mode=-1, m2=0.7, lambda_mode=1, kappa=1.9, Lambda=3.0)
+    phi, pi = _fields(20, 9, phi0=0.3)
+    ophi, opi = O.evolve(m, phi, pi, 15, T0=1.25, dtbase=0.07)
+    dtk, dtd, _ = pn.time_factors(1.25, 0.07, m["a"], 15)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        L.evolve(dtk, dtd)
+        gphi, gpi = L.get_fields()
+    assert np.array_equal(_bits(gphi), _bits(ophi)) and np.array_equal(_bits(gpi), _bits(opi))
+
+
+def test_kick_then_drift_equals_reference_functions():
+    # updateMomenta / updateFields / exchange called one by one
+    N = 16
+    m = O.model(N)
+    phi, pi = _fields(N, 12)
+    dtk, dtd, _ = O.time_factors(4)
+    rphi, rpi = O.to_ref(phi, N), O.to_ref(pi, N)
+    p = O.params(m)
+    import ctypes as C
+    with _lat(m) as L:
+        L.set_fields(rphi, rpi, layout=pn.PN_LAYOUT_REF)
+        for n in range(4):
+            O.lib().or_update_momenta(C.byref(p), O.dptr(rphi), O.dptr(rpi), dtk[n])
+            O.lib().or_update_fields(N, O.dptr(rphi), O.dptr(rpi), dtd[n])
+            O.lib().or_exchange(N, O.dptr(rphi))
+            L.kick(dtk[n])
+            L.drift(dtd[n])
+        gphi, gpi = L.get_fields(layout=pn.PN_LAYOUT_REF)
+    assert np.array_equal(_bits(gphi), _bits(rphi))
+    # the reference never exchanges Pi: compare its interior only
+    P = N * N
+    assert np.array_equal(_bits(gpi[P:P * (N + 1)]), _bits(rpi[P:P * (N + 1)]))
+
+
+def test_layout_roundtrip():
+    N = 12
+    m = O.model(N)
+    phi, pi = _fields(N, 13)
+    with _lat(m) as L:
+        L.set_fields(O.to_ref(phi, N), O.to_ref(pi, N), layout=pn.PN_LAYOUT_REF)
+        dphi, dpi = L.get_fields(layout=pn.PN_LAYOUT_DENSE)
+        rphi, _ = L.get_fields(layout=pn.PN_LAYOUT_REF)
+    assert np.array_equal(dphi, phi) and np.array_equal(dpi, pi)
+    assert np.array_equal(rphi, O.to_ref(phi, N))
+
+
+def test_wrap_unique_values():
+    # a unique value per site: one kick with only the Laplacian must see the
+    # periodic neighbours of every site (comm.cpp:63-78, util.cpp:12-18)
+    N = 4
+    m = dict(N=N, a=1.0, mass_mode=0, m2=0.0, lambda_mode=0, kappa=1e300, Lambda=1.0)
+    phi = np.arange(N ** 3, dtype=np.float64) * 1.0
+    pi = np.zeros(N ** 3)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        L.kick(1.0)
+        _, gpi = L.get_fields()
+    f = phi.reshape(N, N, N)
+    lap = (np.roll(f, 1, 0) + np.roll(f, -1, 0) + np.roll(f, 1, 1) + np.roll(f, -1, 1)
+           + np.roll(f, 1, 2) + np.roll(f, -1, 2) - 6 * f)
+    assert np.array_equal(gpi, lap.ravel())
+
+
+@pytest.mark.parametrize("N,seed", [(8, 21), (16, 22), (48, 23)])
+def test_volume_averages_and_extrema(N, seed):
+    m = O.model(N)
+    phi, pi = _fields(N, seed)
+    ref = O.volume_averages(m, phi, pi)
+    vals, idx = O.extrema(N, phi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        avg = L.volume_averages().as_array()
+        ext = L.extrema()
+    # sums are compensated on the GPU and sequential in the reference: 1e-13
+    np.testing.assert_allclose(avg, ref, rtol=1e-13, atol=1e-15)
+    assert (ext.min, ext.max, ext.maxabs) == tuple(vals)
+    assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
+
+
+def test_extrema_ties_pick_first_site():
+    N = 8
+    m = O.model(N)
+    phi = np.zeros(N ** 3)
+    phi[[100, 7, 300]] = 3.0
+    phi[[50, 400]] = -3.0
+    pi = np.zeros(N ** 3)
+    vals, idx = O.extrema(N, phi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        ext = L.extrema()
+    assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
+    assert (ext.min, ext.max, ext.maxabs) == tuple(vals)
+
+
+@pytest.mark.parametrize("N", [16, 64, 128])
+def test_spectrum_counts_and_sums(N):
+    m = O.model(N)
+    phi, pi = _fields(N, 31 + N)
+    counts = O.bin_table(N, m["a"])
+    F, K, Q = O.spectrum(N, m["a"], phi, pi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        s = L.spectrum()
+    assert s["nbins"] == counts.size
+    assert np.array_equal(s["counts"], counts)
+    np.testing.assert_allclose(s["F"], F, rtol=1e-12, atol=1e-12 * F.max())
+    np.testing.assert_allclose(s["K"], K, rtol=1e-12, atol=1e-12 * K.max())
+    np.testing.assert_allclose(s["Q"], Q, rtol=1e-11, atol=1e-12 * np.abs(Q).max())
+
+
+def test_fft_modes_match_reference_layout():
+    N = 16
+    m = O.model(N)
+    phi, pi = _fields(N, 41)
+    ref = O.ref_modes(phi, N, m["a"])
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        got = L.fft_modes(0)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("N,seed,mode", [(8, 51, 0), (16, 52, 0), (32, 53, 0), (16, 54, 1),
+                                         (32, 55, 1)])
+def test_islands_match_bfs(N, seed, mode):
+    m = O.model(N)
+    phi, pi = _fields(N, seed, phi0=0.0, amp=1.0)
+    if mode == 0:
+        thr = 2.57583 * np.sqrt(O.volume_averages(m, phi, pi)[2])
+    else:
+        thr = 4.0 * O.volume_averages(m, phi, pi)[7]
+    cat, labels = O.islands(m, mode, thr, phi, pi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        rows, glab, _ = L.islands(mode, thr, labels=True)
+    assert len(rows) == cat.shape[0] > 0
+    assert np.array_equal(glab, labels)
+    P = N * N
+    for r, c in zip(rows, cat):
+        assert r.index == int(c[0]) and r.nn == int(c[1])
+        assert r.volume == c[2]
+        assert r.peak_ties == 0
+        assert r.peak_pos == int(c[3])
+        assert r.peak_value == c[4]
+        np.testing.assert_allclose([r.avg_phi, r.pot_en, r.kin_en, r.grad_en], c[5:9],
+                                   rtol=1e-12, atol=1e-14)
+        members = np.nonzero(labels == r.index)[0]
+        assert r.first_pos == members.min() + P
