@@ -105,12 +105,30 @@ typedef struct pn_catalog {
   int *labels;         /* optional, N^3 dense: island index or -1           */
 } pn_catalog;
 
+/* Initial conditions, initialize==3 with withcharge==0 and no expansion
+ * (parameters.cpp:279-303, initialconditions.cpp:45-723). */
+typedef struct pn_ic_params {
+  long long seed;   /* srand48 seed (the reference uses time(NULL)*(ID+1))     */
+  double QS;        /* QS                                                      */
+  double scale2;    /* scale2 (-Q2)                                            */
+  double amplitude; /* Amplitude                                               */
+  double noise;     /* Noise after CompleteParameters (times Lambda)           */
+  double m2_long;   /* M2_long (= M_sqr, renormalization.cpp:17)               */
+  int justnoise;    /* justnoise                                               */
+  double phi0;      /* phi0_0, added after the transform                       */
+  double pi0;       /* pi0_0                                                   */
+} pn_ic_params;
+
 /* ---- context ---------------------------------------------------------- */
 
 /* LocalLattice::init (lattice.cpp:70-104). */
 int pn_create(const pn_params *p, pn_ctx **out);
 /* LocalLattice::fin (lattice.cpp:108-117). */
 int pn_destroy(pn_ctx *c);
+/* Replaces the theory-0 couplings after creation, as fix_tadpole does for
+ * BareM_sqr_long (renormalization.cpp:3-31) and GetParameters for the rest
+ * (parameters.cpp:316-321). */
+int pn_set_couplings(pn_ctx *c, int mass_mode, double m2, int lambda_mode, double kappa);
 /* Text of the last error on this context (or of the last failed pn_create
  * in this thread when c is NULL). */
 const char *pn_last_error(const pn_ctx *c);
@@ -126,6 +144,17 @@ int pn_set_fields(pn_ctx *c, const double *phi, const double *pi, int layout);
 /* Device -> host copy.  PN_LAYOUT_REF also fills the halo planes as
  * Communication::exchange leaves them (comm.cpp:63-78). */
 int pn_get_fields(pn_ctx *c, double *phi, double *pi, int layout);
+
+/* Host part of SetInitialConditions (initialconditions.cpp:45-614): the seeded
+ * drand48 draws per mode in the order pz -> px -> py and the Hermitian
+ * completion of ensureRealResult (clusterfft.cpp:55-145).  A (field) and E
+ * (momentum) receive N*N*(N/2+1) interleaved complex modes in the reference
+ * order py + px*(N/2+1) + pz*N*(N/2+1).  No device work. */
+int pn_ic_modes(int N, double a, const pn_ic_params *ic, double *A, double *E);
+/* SetInitialConditions (initialconditions.cpp:45-723): pn_ic_modes on the
+ * host, rocFFT c2r on the device, /Norm (clusterfft.cpp:150) times NormFactor
+ * (initialconditions.cpp:643) plus the background, into Phi and Pi. */
+int pn_initial_conditions(pn_ctx *c, const pn_ic_params *ic);
 
 /* ---- dynamics (dynamics.cpp) ------------------------------------------ */
 

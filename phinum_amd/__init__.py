@@ -58,6 +58,12 @@ class PnIsland(C.Structure):
                 ("cz", C.c_double), ("peak_ties", C.c_longlong)]
 
 
+class PnIcParams(C.Structure):
+    _fields_ = [("seed", C.c_longlong), ("QS", C.c_double), ("scale2", C.c_double),
+                ("amplitude", C.c_double), ("noise", C.c_double), ("m2_long", C.c_double),
+                ("justnoise", C.c_int), ("phi0", C.c_double), ("pi0", C.c_double)]
+
+
 class PnCatalog(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("threshold", C.c_double),
                 ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
@@ -96,6 +102,8 @@ def lib():
         "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pn_islands": (C.c_int, [C.c_void_p, C.c_int, C.c_double, P(PnCatalog)]),
         "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
+        "pn_ic_modes": (C.c_int, [C.c_int, C.c_double, P(PnIcParams), dp, dp]),
+        "pn_initial_conditions": (C.c_int, [C.c_void_p, P(PnIcParams)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -237,10 +245,38 @@ class Lattice:
         rows = [arr[i] for i in range(cat.count)]
         return rows, lab, cat.threshold
 
+    def initial_conditions(self, ic):
+        """SetInitialConditions (initialize==3) with a PnIcParams."""
+        self._chk(self.L.pn_initial_conditions(self.h, C.byref(ic)))
+
     def energy_density(self):
         rho = np.empty(self.N ** 3)
         self._chk(self.L.pn_energy_density(self.h, _dptr(rho)))
         return rho
+
+
+def ic_params(seed, a=0.5, Lambda=6.0, noise=0.5, m2=1.0, phi0=0.0, pi0=0.0, QS=None,
+              scale2=None, amplitude=0.0, justnoise=0):
+    """initialize==3 parameters; QS and scale2 default to half the lattice
+    cutoff pTmax/2 = sqrt(3)/a (momentumspace.cpp:41); noise is multiplied by
+    Lambda as CompleteParameters does (parameters.cpp:408-410)."""
+    if QS is None:
+        QS = float(np.sqrt(3.0 * (4.0 / (a * a)))) / 2.0
+    if scale2 is None:
+        scale2 = QS
+    return PnIcParams(int(seed), float(QS), float(scale2), float(amplitude), float(noise) * float(Lambda),
+                      float(m2), int(justnoise), float(phi0), float(pi0))
+
+
+def ic_modes(N, a, ic):
+    """Host part of SetInitialConditions: (A, E) modes in [pz][px][py] order."""
+    H = N // 2 + 1
+    A = np.empty(2 * N * N * H)
+    E = np.empty(2 * N * N * H)
+    rc = lib().pn_ic_modes(int(N), float(a), C.byref(ic), _dptr(A), _dptr(E))
+    if rc != PN_OK:
+        raise PhiNumError(rc, lib().pn_last_error(None).decode())
+    return A.view(np.complex128).reshape(N, N, H), E.view(np.complex128).reshape(N, N, H)
 
 
 def time_factors(T0, dtbase, a, nsteps):

@@ -1,0 +1,410 @@
+// phinum_dropin.cpp -- reference-named host layer over the C ABI.
+#include "phinum_dropin.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <stdexcept>
+#include <sys/stat.h>
+
+namespace phinum {
+
+// ---- parameters.cpp defaults (parameters.cpp:10,88-100,172-195,289-301) ----
+int runs = 10, runID = 0;
+int Nc = 1, Mass_Mode = 1, Lambda_Mode = -1;
+double M_sqr = 1., Lambda = 100., Kappa = 2.;
+int renorm_tadpole = 0, max_renSteps = 2000;
+double tad_precision = 1e-6;
+double a_t = 0.5, a_eta = 0.5;
+int N_t = 64, N_eta = 64;
+double T0 = 0, TMax = 100000, TSO = 0.1, TMO = 50., dtBaseTrans = 0.1, dtBaseLong = 0.1;
+int justnoise = 0;
+double QS = 1., scale2 = 1., Amplitude = 0., chi = 1., Noise = 0.5;
+double phi0_0 = 0., phi0_1 = 0., pi0_0 = 0., pi0_1 = 0.;
+double M2_long = 0., M2_trans = 0.;
+int nodes = 1, ID = 0, LASTRUN = 0;
+long long seed = 1234;
+int precision = PN_FP64;
+int islanddetect = 0;
+int island_mode = 0;
+int island_every = 5;
+int write_output = 1;
+
+static std::string g_err;
+std::string last_error() { return g_err; }
+
+static void check(int rc, const char *what) {
+  if (rc != PN_OK) {
+    g_err = std::string(what) + ": " + pn_last_error(LocalLattice::ctx);
+    throw std::runtime_error(g_err);
+  }
+}
+
+// -key value pairs as Konfig (cfile.c:55-73): a later duplicate overwrites,
+// values are read with atof/atoi (cfile.c:252-278), unknown keys are ignored.
+void GetParameters(int argc, char **argv) {
+  std::map<std::string, std::string> kv;
+  for (int i = 1; i < argc; i++)
+    if (argv[i][0] == '-') {
+      std::string key(argv[i] + 1);
+      if (key.size() > 79) key.resize(79);
+      kv[key] = (i + 1 < argc) ? argv[i + 1] : key;
+    }
+  auto gi = [&](const char *k, int &v) { auto it = kv.find(k); if (it != kv.end()) v = std::atoi(it->second.c_str()); };
+  auto gd = [&](const char *k, double &v) { auto it = kv.find(k); if (it != kv.end()) v = std::atof(it->second.c_str()); };
+  gi("runs", runs);
+  gi("Nc", Nc);
+  gd("M", M_sqr);
+  gi("Mmode", Mass_Mode);
+  gd("L", Lambda);
+  gi("Lmode", Lambda_Mode);
+  gd("Kappa", Kappa);
+  gi("tadpole", renorm_tadpole);
+  gd("tadprec", tad_precision);
+  gi("renSteps", max_renSteps);
+  gi("Nt", N_t);
+  gd("at", a_t);
+  gd("T0", T0);
+  gd("TMax", TMax);
+  gd("dt", dtBaseLong);
+  gd("TSO", TSO);
+  gd("TMO", TMO);
+  gi("jn", justnoise);
+  gd("Q2", scale2);
+  gd("Ampl", Amplitude);
+  gd("QS", QS);
+  gd("chi", chi);
+  gd("Noise", Noise);
+  gd("phi0_0", phi0_0);
+  gd("phi0_1", phi0_1);
+  gd("pi0_0", pi0_0);
+  gd("pi0_1", pi0_1);
+  auto it = kv.find("seed");
+  if (it != kv.end()) seed = std::atoll(it->second.c_str());
+  gi("precision", precision);
+  gi("islands", islanddetect);
+  gi("islandmode", island_mode);
+  gi("islandevery", island_every);
+  gi("output", write_output);
+  if (Nc != 1) throw std::runtime_error("GetParameters: the device path evolves one component (Nc=1)");
+  if (renorm_tadpole != 0)
+    throw std::runtime_error("GetParameters: tadpole renormalisation is not implemented upstream either");
+}
+
+void CompleteParameters() {  // parameters.cpp:398-411
+  a_eta = a_t;
+  N_eta = N_t;
+  dtBaseTrans = dtBaseLong;
+  Noise *= Lambda;
+}
+
+// ---- lattice.cpp ----------------------------------------------------------
+namespace LocalLattice {
+double tau = 0, dtau = 0;
+int N_t_loc = 0, N_eta_loc = 0, vol = 0;
+double *Phi = nullptr, *Pi = nullptr;
+double *phi = nullptr, *pi = nullptr;
+double pi2, phi2, phi4, phi6, phipi, Energy, Interaction, Charge;
+double *avgphi = nullptr, *avgpi = nullptr;
+double avgpi2, avgphi2, avgphi4, avgphi6, avgphipi, avgEnergy, avgInteraction, avgCharge;
+double maxPhi, minPhi, maxAbsPhi;
+int maxpos, minpos, maxAbspos;
+double GlmaxPhi, GlminPhi, GlmaxAbsPhi;
+double BareM_sqr_long = 1., BareM_sqr_trans = 1.;
+pn_ctx *ctx = nullptr;
+
+void init() {  // lattice.cpp:70-104
+  N_t_loc = N_t;
+  N_eta_loc = N_eta / nodes;
+  vol = N_t_loc * N_t_loc * N_eta_loc;
+  const size_t n = (size_t)Nc * N_t * N_t * (N_eta_loc + 2);
+  Phi = new double[n]();
+  Pi = new double[n]();
+  phi = new double[Nc]();
+  pi = new double[Nc]();
+  avgphi = new double[Nc]();
+  avgpi = new double[Nc]();
+  BareM_sqr_long = M_sqr;
+  BareM_sqr_trans = M_sqr;
+  pn_params p{N_t, a_t, Mass_Mode, BareM_sqr_long, Lambda_Mode, Kappa, precision, 0};
+  check(pn_create(&p, &ctx), "LocalLattice::init");
+}
+
+void fin() {
+  delete[] Phi;
+  delete[] Pi;
+  delete[] phi;
+  delete[] pi;
+  delete[] avgphi;
+  delete[] avgpi;
+  Phi = Pi = phi = pi = avgphi = avgpi = nullptr;
+  pn_destroy(ctx);
+  ctx = nullptr;
+}
+
+void SyncToHost() { check(pn_get_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToHost"); }
+void SyncToDevice() { check(pn_set_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToDevice"); }
+}  // namespace LocalLattice
+
+// ---- renormalization.cpp / initialconditions.cpp ---------------------------
+void fix_tadpole() {  // renormalization.cpp:3-31 with renorm_tadpole == 0
+  LocalLattice::BareM_sqr_long = M_sqr;
+  LocalLattice::BareM_sqr_trans = M_sqr;
+  M2_long = M_sqr;
+  M2_trans = M_sqr;
+  check(pn_set_couplings(LocalLattice::ctx, Mass_Mode, LocalLattice::BareM_sqr_long, Lambda_Mode, Kappa),
+        "fix_tadpole");
+}
+
+void SetInitialConditions() {
+  pn_ic_params ic;
+  ic.seed = seed + (runID > 0 ? runID - 1 : 0);
+  ic.QS = QS;
+  ic.scale2 = scale2;
+  ic.amplitude = Amplitude;
+  ic.noise = Noise;
+  ic.m2_long = M2_long;
+  ic.justnoise = justnoise;
+  ic.phi0 = phi0_0;
+  ic.pi0 = pi0_0;
+  check(pn_initial_conditions(LocalLattice::ctx, &ic), "SetInitialConditions");
+}
+
+// ---- dynamics.cpp -----------------------------------------------------------
+namespace numeric {
+int regime = 0, total_stepcount = 0, normal_stepcount = 0, adaptive_stepcount = 0;
+double Last_Field_update = 0, Last_Momenta_update = 0, LastMOoutput = 0, LastSOoutput = 0;
+}  // namespace numeric
+
+void reset() {  // dynamics.cpp:28-50
+  using namespace numeric;
+  regime = 0;
+  total_stepcount = normal_stepcount = adaptive_stepcount = 0;
+  LocalLattice::tau = T0;
+  LocalLattice::dtau = dtBaseTrans * a_t;
+  Last_Field_update = LocalLattice::tau;
+  Last_Momenta_update = LocalLattice::tau;
+}
+
+void updateMomenta() {  // dynamics.cpp:52-175
+  check(pn_kick(LocalLattice::ctx, LocalLattice::tau - numeric::Last_Momenta_update), "updateMomenta");
+  numeric::Last_Momenta_update = LocalLattice::tau;
+}
+
+void updateFields() {  // dynamics.cpp:177-194
+  check(pn_drift(LocalLattice::ctx, LocalLattice::tau - numeric::Last_Field_update), "updateFields");
+  numeric::Last_Field_update = LocalLattice::tau;
+}
+
+void evolve() {  // dynamics.cpp:196-237: kick and drift fused into one launch
+  using LocalLattice::tau;
+  tau += 0.5 * LocalLattice::dtau;
+  const double dtk = tau - numeric::Last_Momenta_update;
+  numeric::Last_Momenta_update = tau;
+  tau += 0.5 * LocalLattice::dtau;
+  const double dtd = tau - numeric::Last_Field_update;
+  numeric::Last_Field_update = tau;
+  check(pn_evolve(LocalLattice::ctx, 1, &dtk, &dtd), "evolve");
+  numeric::normal_stepcount++;
+  Communication::exchange();
+  numeric::total_stepcount++;
+}
+
+namespace Communication {
+// comm.cpp:63-78: the periodic z wrap is index arithmetic inside the device
+// kernels, so there is nothing to exchange
+void exchange() {}
+}  // namespace Communication
+
+// ---- fieldanalysis.cpp ----------------------------------------------------
+void CalculateVolumeAverages() {
+  using namespace LocalLattice;
+  pn_averages a;
+  check(pn_volume_averages(ctx, &a), "CalculateVolumeAverages");
+  phi[0] = avgphi[0] = a.phi;
+  pi[0] = avgpi[0] = a.pi;
+  phi2 = avgphi2 = a.phi2;
+  pi2 = avgpi2 = a.pi2;
+  phipi = avgphipi = a.phipi;
+  phi4 = avgphi4 = a.phi4;
+  phi6 = avgphi6 = a.phi6;
+  Energy = avgEnergy = a.energy;
+  Interaction = avgInteraction = a.interaction;
+  Charge = avgCharge = a.charge;
+}
+
+void CalculateExtrema() {
+  using namespace LocalLattice;
+  pn_extrema e;
+  check(pn_extrema_get(ctx, &e), "CalculateExtrema");
+  GlminPhi = minPhi = e.min;
+  GlmaxPhi = maxPhi = e.max;
+  GlmaxAbsPhi = maxAbsPhi = e.maxabs;
+  minpos = (int)e.minpos;
+  maxpos = (int)e.maxpos;
+  maxAbspos = (int)e.maxabspos;
+}
+
+static std::ofstream SpatialObs;
+
+void Prepare_Output() {  // fieldanalysis.cpp:48-79
+  if (!write_output || ID != 0) return;
+  {
+    std::ifstream fin("lastrun.txt");
+    fin >> LASTRUN;
+  }
+  {
+    std::ofstream fout("lastrun.txt");
+    fout << ++LASTRUN << std::endl;
+  }
+  char folder[256];
+  std::snprintf(folder, sizeof folder, "./output/%i", LASTRUN);
+  struct stat info;
+  if (stat(folder, &info) == 0 && S_ISDIR(info.st_mode))
+    std::cout << "Warning: Output folder " << folder << " already exists! Files will be overwritten." << std::endl;
+  mkdir("./output", 0755);
+  mkdir(folder, 0755);
+}
+
+void Print_Info_Console() {  // fieldanalysis.cpp:81-92 (theory 0)
+  if (ID != 0) return;
+  std::cout << "Parameters" << std::endl;
+  std::cout << "Nc=" << Nc << std::endl;
+  std::cout << "m^2=" << Mass_Mode * M_sqr << std::endl;
+  std::cout << "Lambda=" << Lambda_Mode * Lambda << std::endl;
+  std::cout << "Nyu=" << (Lambda / Kappa) * (Lambda / Kappa) << std::endl;
+}
+
+void Print_AveragesExtrema() {  // fieldanalysis.cpp:748-799
+  using namespace LocalLattice;
+  if (numeric::total_stepcount > 0 && tau < numeric::LastSOoutput + TSO) return;
+  if (numeric::total_stepcount == 0) {
+    if (ID == 0 && write_output) {
+      if (SpatialObs.is_open()) SpatialObs.close();
+      char fname[256];
+      std::snprintf(fname, sizeof fname, "./output/%i/SO-%i.txt", LASTRUN, runID);
+      SpatialObs.open(fname, std::fstream::app);
+    }
+    numeric::LastSOoutput = tau;
+  } else if (tau >= numeric::LastSOoutput + TSO) {
+    numeric::LastSOoutput += TSO;
+  }
+  CalculateExtrema();
+  CalculateVolumeAverages();
+  if (ID == 0 && write_output && SpatialObs.is_open()) {
+    SpatialObs << numeric::LastSOoutput << ' ';
+    for (int i = 0; i < Nc; i++) SpatialObs << avgphi[i] << ' ' << avgpi[i] << ' ';
+    SpatialObs << avgphi2 << ' ' << avgpi2 << ' ' << avgphipi << ' ' << avgphi4 << ' ' << avgphi6 << ' ';
+    SpatialObs << avgEnergy << ' ' << avgInteraction << ' ' << avgCharge << ' ';
+    SpatialObs << GlminPhi << ' ' << GlmaxPhi << ' ' << GlmaxAbsPhi << std::endl;
+  }
+}
+
+bool false_vacuum_decayed() {  // fieldanalysis.cpp:887-893
+  const double I = LocalLattice::avgInteraction;
+  return (I < 0 || I != I || I * 0. != 0.);
+}
+
+// ---- momentumspace.cpp ------------------------------------------------------
+namespace MomentumSpace {
+double pTmin = 0, pTmax = 0, binsizeT = 0;
+int nTBins = 0;
+std::vector<double> TotalCorrF, TotalCorrK, TotalCorrQ;
+std::vector<int> TotalBCounts;
+void init() {  // momentumspace.cpp:32-65
+  pTmin = (2.0 / (a_t)) * std::sin(std::acos(-1.) / N_t);
+  pTmax = std::sqrt(3.0 * (4.0 / (a_t * a_t)));
+  binsizeT = pTmin;
+  nTBins = int(pTmax / binsizeT + 0.5) + 2;
+  TotalCorrF.assign(nTBins, 0.);
+  TotalCorrK.assign(nTBins, 0.);
+  TotalCorrQ.assign(nTBins, 0.);
+  TotalBCounts.assign(nTBins, 0);
+}
+void fin() {
+  TotalCorrF.clear();
+  TotalCorrK.clear();
+  TotalCorrQ.clear();
+  TotalBCounts.clear();
+}
+}  // namespace MomentumSpace
+
+void Calculate_CorrelationFunctions() {  // momentumspace.cpp:140-278
+  using namespace MomentumSpace;
+  pn_spectrum s;
+  s.nbins = nTBins;
+  s.counts = TotalBCounts.data();
+  s.corr_f = TotalCorrF.data();
+  s.corr_k = TotalCorrK.data();
+  s.corr_q = TotalCorrQ.data();
+  check(pn_spectrum_get(LocalLattice::ctx, &s), "Calculate_CorrelationFunctions");
+}
+
+void Print_fp() {  // momentumspace.cpp:280-384
+  using namespace LocalLattice;
+  using namespace MomentumSpace;
+  if (numeric::total_stepcount > 0 && tau < numeric::LastMOoutput + TMO) return;
+  if (numeric::total_stepcount == 0) numeric::LastMOoutput = tau;
+  else if (tau >= numeric::LastMOoutput + TMO) numeric::LastMOoutput += TMO;
+  Calculate_CorrelationFunctions();
+  if (ID != 0 || !write_output) return;
+  char fname[256];
+  std::snprintf(fname, sizeof fname, "./output/%i/Distrib-%i-%g.txt", LASTRUN, runID,
+                double(numeric::LastMOoutput));
+  std::ofstream specOut(fname);
+  double NormCorr = 1. / ((N_t * a_t) * (N_t * a_t) * N_eta * a_eta);
+  for (int k = 0; k < nTBins; k++) {
+    double norm2 = (TotalBCounts[k] == 0 ? 0 : 1 / (double)(TotalBCounts[k]));
+    double pabs = binsizeT * k;
+    if (norm2 != 0 && TotalCorrF[k * Nc] != 0) {
+      if (k != 0) specOut << std::endl;
+      specOut << pabs << " ";
+      specOut << TotalBCounts[k] << " ";
+      double FF = TotalCorrF[k] / Nc, KK = TotalCorrK[k] / Nc, QQ = TotalCorrQ[k] / Nc;
+      double omg = std::sqrt(KK / FF);
+      specOut << omg << ' ';
+      specOut << FF * NormCorr * norm2 << ' ' << KK * NormCorr * norm2 << ' '
+              << std::sqrt(FF * KK) * NormCorr * norm2 << " ";
+      specOut << FF * NormCorr * norm2 * pabs * pabs * pabs / 2. / std::acos(-1.) / std::acos(-1.) << ' ';
+      if (FF * KK - QQ * QQ < 0) std::cout << "negative at " << k << std::endl;
+    }
+  }
+}
+
+// ---- islands.cpp -------------------------------------------------------------
+namespace island_analysis {
+double thres = 1.125;
+int number_of_islands = 0;
+std::vector<pn_island> found_islands;
+void init() { number_of_islands = 0; found_islands.clear(); }
+void fin() { found_islands.clear(); }
+}  // namespace island_analysis
+
+// Catalog of the islands present now (islands.cpp:778-840 on a lattice with
+// no tracked islands): numbering follows the raster scan of :827-835.
+void Full_Analysis() {
+  using namespace island_analysis;
+  if (island_mode == 1) {  // islands.cpp:783-786
+    CalculateVolumeAverages();
+    thres = LocalLattice::avgEnergy * 4.;
+  }
+  std::vector<pn_island> buf(4096);
+  pn_catalog cat{(int)buf.size(), 0, 0.0, buf.data(), nullptr};
+  int rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
+  if (rc == PN_ERR_CAPACITY) {
+    buf.resize(cat.count);
+    cat.capacity = cat.count;
+    cat.islands = buf.data();
+    rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
+  }
+  check(rc, "Full_Analysis");
+  buf.resize(cat.count);
+  found_islands = buf;
+  number_of_islands = cat.count;
+}
+
+}  // namespace phinum

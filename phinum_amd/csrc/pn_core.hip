@@ -169,6 +169,73 @@ __global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *
   }
 }
 
+// Vector form: each thread owns W consecutive x sites of one row (16-byte
+// loads and stores: W = 2 doubles or 4 floats), a wave covers 64*W sites of
+// the row.  Neighbours inside the vector come from registers; only the two
+// outer x neighbours are separate loads.  Needs N % W == 0.
+template <typename T, int W>
+struct VecT {
+  typedef T type __attribute__((ext_vector_type(W)));
+};
+
+template <typename T, typename C, bool POW2, int W, int BY, int ZB, int MODE>
+__global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi, T *__restrict__ pi,
+                                                      T *__restrict__ phin, StepGeom g, C k,
+                                                      T dtk, T dtd) {
+  typedef typename VecT<T, W>::type V;
+  const int nblocks = g.XT * g.YT * g.ZT;
+  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int xt = t % g.XT;
+  const int yt = (t / g.XT) % g.YT;
+  const int zt = t / (g.XT * g.YT);
+  const int N = g.N;
+  const int x = xt * 64 * W + W * (int)threadIdx.x;
+  const int y = yt * BY + (int)threadIdx.y;
+  if (x >= N || y >= N) return;
+  const int z0 = zt * ZB;
+  const int z1 = min(z0 + ZB, N);
+  const int xp = (x + W == N) ? 0 : x + W;
+  const int xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1;
+  const int ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  const int zm0 = (z0 == 0) ? N - 1 : z0 - 1;
+  V fm = *(const V *)&phi[zm0 * P + row + x];
+  V fc = *(const V *)&phi[z0 * P + row + x];
+  for (int z = z0; z < z1; z++) {
+    const int zp = (z + 1 == N) ? 0 : z + 1;
+    const long long base = z * P;
+    const V fp = *(const V *)&phi[zp * P + row + x];
+    const T vxp = phi[base + row + xp];
+    const T vxm = phi[base + row + xm];
+    const V vyp = *(const V *)&phi[base + rowp + x];
+    const V vym = *(const V *)&phi[base + rowm + x];
+    V p = __builtin_nontemporal_load((const V *)&pi[base + row + x]);
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const T right = (j == W - 1) ? vxp : fc[(j + 1) % W];
+      const T left = (j == 0) ? vxm : fc[(j + W - 1) % W];
+      T pj = p[j];
+      if constexpr (sizeof(T) == 8) {
+        site_update_f64<POW2>(k, right, left, vyp[j], vym[j], fp[j], fm[j], fc[j], pj, dtk);
+      } else {
+        site_update_f32(k, right, left, vyp[j], vym[j], fp[j], fm[j], fc[j], pj, dtk);
+      }
+      p[j] = pj;
+    }
+    __builtin_nontemporal_store(p, (V *)&pi[base + row + x]);
+    if constexpr (MODE == 0) {
+      V fnew;
+#pragma unroll
+      for (int j = 0; j < W; j++) fnew[j] = fc[j] + p[j] * dtd;
+      __builtin_nontemporal_store(fnew, (V *)&phin[base + row + x]);
+    }
+    fm = fc;
+    fc = fp;
+  }
+}
+
 // updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.
 template <typename T>
 __global__ void k_drift(T *__restrict__ phi, const T *__restrict__ pi, long long n, T dtd) {
@@ -183,10 +250,29 @@ static constexpr int kZB = 16;
 
 template <typename T, typename C, bool POW2, int MODE>
 static void launch_step_t(pn_ctx *c, const C &k, T dtk, T dtd) {
-  StepGeom g = step_geom(c->p.N, kBY, kZB);
-  int nblocks = g.XT * g.YT * g.ZT;
+  constexpr int W = 16 / sizeof(T);
+  const int N = c->p.N;
   const T *phi = (const T *)c->phi[c->cur];
   T *phin = (T *)c->phi[c->cur ^ 1];
+  if (N % W == 0) {
+    // z blocks of 32 planes below 512^3 keep enough tiles in flight; 16 above
+    if (N < 512) {
+      StepGeom g = step_geom(N, kBY, 32);
+      g.XT = (N + 64 * W - 1) / (64 * W);
+      const int nblocks = g.XT * g.YT * g.ZT;
+      hipLaunchKernelGGL((k_step_vec<T, C, POW2, W, kBY, 32, MODE>), dim3(nblocks), dim3(64, kBY),
+                         0, c->stream, phi, (T *)c->pi, phin, g, k, dtk, dtd);
+    } else {
+      StepGeom g = step_geom(N, kBY, kZB);
+      g.XT = (N + 64 * W - 1) / (64 * W);
+      const int nblocks = g.XT * g.YT * g.ZT;
+      hipLaunchKernelGGL((k_step_vec<T, C, POW2, W, kBY, kZB, MODE>), dim3(nblocks), dim3(64, kBY),
+                         0, c->stream, phi, (T *)c->pi, phin, g, k, dtk, dtd);
+    }
+    return;
+  }
+  StepGeom g = step_geom(N, kBY, kZB);
+  int nblocks = g.XT * g.YT * g.ZT;
   hipLaunchKernelGGL((k_step<T, C, POW2, kBY, kZB, MODE>), dim3(nblocks), dim3(64, kBY), 0,
                      c->stream, phi, (T *)c->pi, phin, g, k, dtk, dtd);
 }
@@ -305,6 +391,20 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
     return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: zero fill failed"));
   c->cur = 0;
   *out = c;
+  return PN_OK;
+}
+
+extern "C" int pn_set_couplings(pn_ctx *c, int mass_mode, double m2, int lambda_mode, double kappa) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_set_couplings: NULL context");
+  if (!(kappa != 0) || !std::isfinite(kappa))
+    return pn_fail(c, PN_ERR_ARG, "pn_set_couplings: Kappa must be finite and nonzero");
+  c->p.mass_mode = mass_mode;
+  c->p.m2 = m2;
+  c->p.lambda_mode = lambda_mode;
+  c->p.kappa = kappa;
+  c->k.c2 = mass_mode * m2;
+  c->k.lmode = (double)lambda_mode;
+  c->k.kappa = kappa;
   return PN_OK;
 }
 

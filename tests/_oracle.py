@@ -211,7 +211,7 @@ def ic_modes(N, a, seed, m2_long=1.0, Lambda=6.0, noise=0.5, QS=None, scale2=Non
 
 def c2r_unnormalised(F, N):
     """FFTW-style unnormalised c2r of [pz][px][py] modes -> Out[x + y*N + z*N*N]."""
-    out_zxy = np.fft.irfftn(F, s=(N, N, N)) * float(N) ** 3   # [z][x][y]
+    out_zxy = np.fft.irfftn(F, s=(N, N, N), axes=(0, 1, 2)) * float(N) ** 3   # [z][x][y]
     return np.ascontiguousarray(out_zxy.transpose(0, 2, 1)).ravel()   # [z][y][x]
 
 

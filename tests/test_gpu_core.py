@@ -51,6 +51,40 @@ def test_non_pow2_spacing_and_other_couplings_bitwise():
     assert np.array_equal(_bits(gphi), _bits(ophi)) and np.array_equal(_bits(gpi), _bits(opi))
 
 
+@pytest.mark.parametrize("N,steps,seed", [(16, 20, 61), (18, 10, 62), (64, 10, 63), (100, 4, 64)])
+def test_fused_step_bitwise_fp32(N, steps, seed):
+    # the FP32 form against its binary32 oracle (vector path when N % 4 == 0)
+    m = O.model(N)
+    phi, pi = _fields(N, seed)
+    ophi, opi = O.evolve_f32(m, phi, pi, steps)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    with _lat(m, pn.PN_FP32) as L:
+        L.set_fields(phi, pi)
+        L.evolve(dtk, dtd)
+        gphi, gpi = L.get_fields()
+    assert np.array_equal(gphi.astype(np.float32), ophi)
+    assert np.array_equal(gpi.astype(np.float32), opi)
+
+
+def test_fp32_energy_tracks_fp64():
+    # FP32 fields with FP64 compensated sums: relative energy difference to
+    # the FP64 run stays below 1e-6 over 2000 steps at 32^3
+    N = 32
+    m = O.model(N)
+    phi, pi = _fields(N, 65, amp=0.2)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 2000)
+    with _lat(m) as L64, _lat(m, pn.PN_FP32) as L32:
+        L64.set_fields(phi, pi)
+        L32.set_fields(phi, pi)
+        for c in range(10):
+            sl = slice(200 * c, 200 * (c + 1))
+            L64.evolve(dtk[sl], dtd[sl])
+            L32.evolve(dtk[sl], dtd[sl])
+            e64 = L64.volume_averages().energy
+            e32 = L32.volume_averages().energy
+            assert abs(e32 - e64) <= 1e-6 * abs(e64)
+
+
 def test_kick_then_drift_equals_reference_functions():
     # updateMomenta / updateFields / exchange called one by one
     N = 16

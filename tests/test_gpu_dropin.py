@@ -1,0 +1,93 @@
+"""The reference-named host layer and driver (phinum_main, main.cpp:21-117)
+against the oracle, plus SetInitialConditions on the device."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import phinum_amd as pn
+
+pytestmark = pytest.mark.gpu
+MAIN = os.path.join(os.path.dirname(pn.__file__), "phinum_main")
+
+
+@pytest.mark.parametrize("N,seed,prec", [(16, 1234, pn.PN_FP64), (32, 7, pn.PN_FP64),
+                                         (16, 3, pn.PN_FP32)])
+def test_initial_conditions_match_oracle(N, seed, prec):
+    m = O.model(N)
+    phi0 = float(np.sqrt(6.0))
+    ophi, opi = O.initial_conditions(m, seed, phi0)
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
+                    precision=prec) as L:
+        L.initial_conditions(pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=phi0))
+        gphi, gpi = L.get_fields()
+    # rocFFT and numpy's c2r differ in rounding only
+    tol = 1e-12 if prec == pn.PN_FP64 else 2e-7
+    assert np.max(np.abs(gphi - ophi)) <= tol * np.max(np.abs(ophi))
+    assert np.max(np.abs(gpi - opi)) <= tol * np.max(np.abs(opi))
+    assert np.std(gphi) > 0.01  # fluctuations are there
+
+
+def _cadence(T0, TMax, dtbase, a, TSO):
+    """Step indices and LastSOoutput values at which Print_AveragesExtrema writes
+    (fieldanalysis.cpp:753-780) inside for(reset(); tau<TMax; evolve())."""
+    dtau = dtbase * a
+    tau = T0
+    last = None
+    out = []
+    step = 0
+    while tau < TMax:
+        if step == 0:
+            last = tau
+            out.append((step, last))
+        elif not tau < last + TSO:
+            if tau >= last + TSO:
+                last = last + TSO
+            out.append((step, last))
+        tau = tau + 0.5 * dtau
+        tau = tau + 0.5 * dtau
+        step += 1
+    return out
+
+
+def test_driver_matches_oracle(tmp_path):
+    N, seed = 16, 1234
+    m = O.model(N)
+    phi0 = float(np.sqrt(6.0))
+    args = [MAIN, "-runs", "1", "-Nt", str(N), "-at", "0.5", "-M", "1", "-Mmode", "1", "-L", "6",
+            "-Lmode", "-1", "-Kappa", "%.17g" % m["kappa"], "-T0", "0", "-TMax", "2.0", "-dt", "0.1",
+            "-TSO", "0.5", "-TMO", "1.0", "-Noise", "0.5", "-Ampl", "0", "-QS", "%.17g" % (np.sqrt(3.0) / 0.5),
+            "-Q2", "%.17g" % (np.sqrt(3.0) / 0.5), "-phi0_0", "%.17g" % phi0, "-seed", str(seed), "-islands", "1"]
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    so = np.loadtxt(tmp_path / "output" / "1" / "SO-1.txt")
+    cad = _cadence(0.0, 2.0, 0.1, 0.5, 0.5)
+    assert so.shape[0] == len(cad)
+    m2 = dict(m)
+    ophi, opi = O.initial_conditions(m2, seed, phi0)
+    prev = 0
+    for (step, last), row in zip(cad, so):
+        if step > prev:
+            ophi, opi = _cont(m, ophi, opi, prev, step)
+            prev = step
+        avg = O.volume_averages(m, ophi, opi)
+        vals, _ = O.extrema(N, ophi)
+        ref = np.concatenate([[last], avg, vals])
+        np.testing.assert_allclose(row, ref, rtol=2e-5, atol=1e-9)
+    dist = sorted(os.listdir(tmp_path / "output" / "1"))
+    assert any(d.startswith("Distrib-1-") for d in dist)
+
+
+def _cont(m, phi, pi, start, stop):
+    """oracle steps start..stop-1 with the factors those steps get in one run"""
+    dtk, dtd, _ = O.time_factors(stop)
+    import ctypes as C
+    N = m["N"]
+    rphi, rpi = O.to_ref(phi, N), O.to_ref(pi, N)
+    p = O.params(m)
+    k = np.ascontiguousarray(dtk[start:stop])
+    d = np.ascontiguousarray(dtd[start:stop])
+    O.lib().or_evolve(C.byref(p), O.dptr(rphi), O.dptr(rpi), stop - start, O.dptr(k), O.dptr(d))
+    return O.from_ref(rphi, N), O.from_ref(rpi, N)
