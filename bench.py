@@ -89,7 +89,20 @@ def main():
     ap.add_argument("--out-every", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB (to scale rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 8,
+                          "copy_calibration_bytes_written": b.numel() * 8, "dispatches": 10}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the lattice fits one MI355X; "
               "multi-GPU z-slab sharding is a design, not measured here)", file=sys.stderr)

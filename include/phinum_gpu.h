@@ -201,6 +201,48 @@ int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out);
  * the summand of Energy (fieldanalysis.cpp:625), dense order. */
 int pn_energy_density(pn_ctx *c, double *rho);
 
+/* ---- batched realizations (no reference counterpart: the reference runs
+ * `runs` realizations one after another, main.cpp:52-96) ------------------ */
+
+/* One realization of a batch: size, spacing and theory-0 couplings. */
+typedef struct pn_realization {
+  int N;
+  double a;
+  int mass_mode;
+  double m2;
+  int lambda_mode;
+  double kappa;
+} pn_realization;
+
+typedef struct pn_batch pn_batch;
+
+/* R realizations, sizes may differ (N must be a multiple of 2 for FP64 and of
+ * 4 for FP32).  Fields start at zero. */
+int pn_batch_create(int R, const pn_realization *reals, int precision, int device, pn_batch **out);
+int pn_batch_destroy(pn_batch *b);
+const char *pn_batch_last_error(const pn_batch *b);
+/* A pn_ctx aliasing realization r, owned by the batch (do not pn_destroy it).
+ * Every analysis and field call of this header works on it (initial
+ * conditions, fields, averages, extrema, spectrum, islands); stepping it
+ * alone returns PN_ERR_STATE. */
+int pn_batch_view(pn_batch *b, int r, pn_ctx **view);
+/* Advances every realization by nsteps leapfrog steps in one launch per
+ * step; dtk and dtd are [R][nsteps] tables (each realization's own replay of
+ * dynamics.cpp:196-237).  use_graph replays captured 100-step HIP graphs. */
+int pn_batch_evolve(pn_batch *b, int nsteps, const double *dtk, const double *dtd, int use_graph);
+int pn_batch_evolve_timed(pn_batch *b, int nsteps, const double *dtk, const double *dtd,
+                          int use_graph, double *ms);
+int pn_batch_sync(pn_batch *b);
+/* Freezes the flagged realizations (the false_vacuum_decayed break of
+ * main.cpp:85-89): their fields stop changing. flags: R ints. */
+int pn_batch_set_decayed(pn_batch *b, const int *flags);
+/* Tile shape for every realization of size N: rows per tile, planes per z block. */
+int pn_batch_set_tiles(pn_batch *b, int N, int rows, int zblock);
+/* Times candidate tile shapes per size class on the live batch (state is
+ * restored afterwards) and keeps the fastest.  out receives nclasses triples
+ * (N, rows, zblock); pass NULL to skip. */
+int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nclasses);
+
 #ifdef __cplusplus
 }
 #endif

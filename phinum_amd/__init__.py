@@ -64,6 +64,11 @@ class PnIcParams(C.Structure):
                 ("justnoise", C.c_int), ("phi0", C.c_double), ("pi0", C.c_double)]
 
 
+class PnRealization(C.Structure):
+    _fields_ = [("N", C.c_int), ("a", C.c_double), ("mass_mode", C.c_int), ("m2", C.c_double),
+                ("lambda_mode", C.c_int), ("kappa", C.c_double)]
+
+
 class PnCatalog(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("threshold", C.c_double),
                 ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
@@ -104,6 +109,17 @@ def lib():
         "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
         "pn_ic_modes": (C.c_int, [C.c_int, C.c_double, P(PnIcParams), dp, dp]),
         "pn_initial_conditions": (C.c_int, [C.c_void_p, P(PnIcParams)]),
+        "pn_set_couplings": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
+        "pn_batch_create": (C.c_int, [C.c_int, P(PnRealization), C.c_int, C.c_int, P(C.c_void_p)]),
+        "pn_batch_destroy": (C.c_int, [C.c_void_p]),
+        "pn_batch_last_error": (C.c_char_p, [C.c_void_p]),
+        "pn_batch_view": (C.c_int, [C.c_void_p, C.c_int, P(C.c_void_p)]),
+        "pn_batch_evolve": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int]),
+        "pn_batch_evolve_timed": (C.c_int, [C.c_void_p, C.c_int, dp, dp, C.c_int, dp]),
+        "pn_batch_sync": (C.c_int, [C.c_void_p]),
+        "pn_batch_set_decayed": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "pn_batch_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+        "pn_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -142,6 +158,18 @@ class Lattice:
         if rc != PN_OK:
             raise PhiNumError(rc, self.L.pn_last_error(None).decode())
         self.h = h
+        self._owned = True
+
+    @classmethod
+    def _borrowed(cls, handle, N, a):
+        """Wraps a context owned elsewhere (a batch view); close() leaves it."""
+        obj = cls.__new__(cls)
+        obj.L = lib()
+        obj.N = int(N)
+        obj.a = float(a)
+        obj.h = handle
+        obj._owned = False
+        return obj
 
     def _chk(self, rc):
         if rc != PN_OK:
@@ -150,8 +178,13 @@ class Lattice:
 
     def close(self):
         if getattr(self, "h", None):
-            self.L.pn_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.L.pn_destroy(self.h)
             self.h = None
+
+    def set_couplings(self, mass_mode, m2, lambda_mode, kappa):
+        self._chk(self.L.pn_set_couplings(self.h, int(mass_mode), float(m2), int(lambda_mode),
+                                          float(kappa)))
 
     def __del__(self):
         try:
@@ -302,3 +335,88 @@ def time_factors(T0, dtbase, a, nsteps):
         last_f = tau
         taus[n] = tau
     return dtk, dtd, taus
+
+
+class Batch:
+    """Many lattices stepped together (pn_batch_*); views give per-realization
+    access through the Lattice interface."""
+
+    def __init__(self, realizations, precision=PN_FP64, device=0):
+        self.L = lib()
+        self.reals = [dict(r) for r in realizations]
+        arr = (PnRealization * len(self.reals))()
+        for i, r in enumerate(self.reals):
+            arr[i] = PnRealization(int(r["N"]), float(r.get("a", 0.5)), int(r.get("mass_mode", 1)),
+                                   float(r.get("m2", 1.0)), int(r.get("lambda_mode", -1)),
+                                   float(r["kappa"]))
+        h = C.c_void_p()
+        rc = self.L.pn_batch_create(len(self.reals), arr, int(precision), int(device), C.byref(h))
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_batch_last_error(None).decode())
+        self.h = h
+        self.R = len(self.reals)
+
+    def _chk(self, rc):
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def view(self, r):
+        v = C.c_void_p()
+        self._chk(self.L.pn_batch_view(self.h, int(r), C.byref(v)))
+        rr = self.reals[r]
+        return Lattice._borrowed(v, rr["N"], rr.get("a", 0.5))
+
+    def _tables(self, dtk, dtd):
+        dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+        dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+        if dtk.ndim == 1:
+            dtk = np.ascontiguousarray(np.broadcast_to(dtk, (self.R, dtk.size)))
+            dtd = np.ascontiguousarray(np.broadcast_to(dtd, (self.R, dtd.size)))
+        if dtk.shape != dtd.shape or dtk.shape[0] != self.R:
+            raise ValueError("dtk/dtd must be [R][nsteps]")
+        return dtk, dtd
+
+    def evolve(self, dtk, dtd, use_graph=True, sync=True):
+        dtk, dtd = self._tables(dtk, dtd)
+        self._chk(self.L.pn_batch_evolve(self.h, int(dtk.shape[1]), _dptr(dtk), _dptr(dtd),
+                                         int(bool(use_graph))))
+        if sync:
+            self._chk(self.L.pn_batch_sync(self.h))
+
+    def evolve_timed(self, dtk, dtd, use_graph=True):
+        dtk, dtd = self._tables(dtk, dtd)
+        ms = C.c_double()
+        self._chk(self.L.pn_batch_evolve_timed(self.h, int(dtk.shape[1]), _dptr(dtk), _dptr(dtd),
+                                               int(bool(use_graph)), C.byref(ms)))
+        return ms.value
+
+    def set_decayed(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        self._chk(self.L.pn_batch_set_decayed(self.h, f.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def set_tiles(self, N, rows, zblock):
+        self._chk(self.L.pn_batch_set_tiles(self.h, int(N), int(rows), int(zblock)))
+
+    def autotune(self, trial_steps=10):
+        out = np.zeros(3 * 64, dtype=np.int32)
+        n = C.c_int()
+        self._chk(self.L.pn_batch_autotune(self.h, int(trial_steps),
+                                           out.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)))
+        return [tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n.value)]

@@ -1,0 +1,544 @@
+// pn_batch.hip -- many independent lattices (seeds, couplings and sizes may
+// differ) advanced together.
+//
+// Layout: the realizations' fields are packed one after the other in three
+// device buffers (Phi ping-pong and Pi), each realization dense x-fastest as
+// in the single-lattice path.  A descriptor per realization carries its
+// offset, size, couplings, tile shape and a decayed flag.
+//
+// Step: one launch per leapfrog step for all realizations.  The launch walks
+// a tile list that concatenates every realization's (x, y, z) tiles; blocks
+// loop over it with a grid stride, so the grid stays a fixed multiple of the
+// CU count whatever the mix of sizes.  The per-site arithmetic is the same
+// device function as the single-lattice kernel (bit-identical results).
+// Sequences of steps are captured once into a HIP graph and replayed.
+//
+// Tile shape (rows per tile, planes per z block) is chosen per size class by
+// an autotuner that times the candidates on the live batch and restores it.
+//
+// Analysis (averages, extrema, spectra, islands, IC) goes through views: a
+// pn_ctx that aliases one realization's slice of the buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pn_internal.h"
+#include "pn_kernels.h"
+
+struct BatchDesc {
+  long long off;   // element offset of this realization
+  int N, XT, YT, ZT, TY, ZB;
+  int tile_start;  // first tile of this realization in the tile list
+  int pow2, decayed;
+  double a2, inv_a2, c2, lmode, kappa;
+  float f_inv_a2, f_c2, f_c4, f_c6;
+};
+
+struct pn_batch {
+  int R = 0, precision = PN_FP64, device = 0;
+  size_t esize = 8;
+  std::vector<pn_realization> re;
+  std::vector<long long> off;
+  long long total = 0;
+  void *phi[2] = {nullptr, nullptr};
+  void *pi = nullptr;
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  std::vector<BatchDesc> hdesc;
+  BatchDesc *ddesc = nullptr;
+  int *dtile = nullptr;  // R+1 tile starts
+  int total_tiles = 0;
+  std::vector<int> htile;
+  double *dtk = nullptr, *dtd = nullptr;  // [R][cap]
+  int table_cap = 0;
+  int *dbase = nullptr;  // step base for graph replays
+  std::map<int, std::pair<int, int>> tiles;  // N -> (TY, ZB)
+  std::vector<pn_ctx *> views;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int graph_len = 0;
+  std::string err;
+};
+
+static int bfail(pn_batch *b, int code, const char *msg, const char *extra = "") {
+  if (b) b->err = std::string(msg) + extra;
+  pn_set_thread_error((std::string(msg) + extra).c_str());
+  return code;
+}
+
+#define PB_HIP(b, call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return bfail((b), PN_ERR_HIP, #call ": ", hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kernel
+// ---------------------------------------------------------------------------
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_batch_step(const T *__restrict__ phi_all,
+                                                    T *__restrict__ pi_all,
+                                                    T *__restrict__ phin_all,
+                                                    const BatchDesc *__restrict__ desc,
+                                                    const int *__restrict__ tile_start, int R,
+                                                    int total_tiles, const double *__restrict__ dtk_tab,
+                                                    const double *__restrict__ dtd_tab, int cap,
+                                                    const int *__restrict__ base_ptr, int j) {
+  typedef typename VecT<T, W>::type V;
+  const int step = *base_ptr + j;
+  for (int b = blockIdx.x; b < total_tiles; b += gridDim.x) {
+    // realization owning tile b: last r with tile_start[r] <= b
+    int lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tile_start[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    const BatchDesc d = desc[lo];
+    const int t = b - d.tile_start;
+    const int xt = t % d.XT, yt = (t / d.XT) % d.YT, zt = t / (d.XT * d.YT);
+    const int N = d.N;
+    const T *phi = phi_all + d.off;
+    T *pi = pi_all + d.off;
+    T *phin = phin_all + d.off;
+    const int x = xt * 64 * W + W * (int)threadIdx.x;
+    const int z0 = zt * d.ZB, z1 = min(z0 + d.ZB, N);
+    const T dk = (T)dtk_tab[(long long)lo * cap + step];
+    const T dd = (T)dtd_tab[(long long)lo * cap + step];
+    if (x >= N) continue;
+    for (int yy = (int)threadIdx.y; yy < d.TY; yy += blockDim.y) {
+      const int y = yt * d.TY + yy;
+      if (y >= N) break;
+      const int xp = (x + W == N) ? 0 : x + W;
+      const int xm = (x == 0) ? N - 1 : x - 1;
+      const int yp = (y + 1 == N) ? 0 : y + 1;
+      const int ym = (y == 0) ? N - 1 : y - 1;
+      const long long P = (long long)N * N;
+      const int row = y * N, rowp = yp * N, rowm = ym * N;
+      const int zm0 = (z0 == 0) ? N - 1 : z0 - 1;
+      V fm = *(const V *)&phi[zm0 * P + row + x];
+      V fc = *(const V *)&phi[z0 * P + row + x];
+      for (int z = z0; z < z1; z++) {
+        const int zp = (z + 1 == N) ? 0 : z + 1;
+        const long long base = z * P;
+        const V fp = *(const V *)&phi[zp * P + row + x];
+        if (d.decayed) {
+          // a decayed realization is frozen (main.cpp:85-89 ends its run);
+          // the copy keeps both ping-pong buffers holding its last field
+          __builtin_nontemporal_store(fc, (V *)&phin[base + row + x]);
+          fm = fc;
+          fc = fp;
+          continue;
+        }
+        const T vxp = phi[base + row + xp];
+        const T vxm = phi[base + row + xm];
+        const V vyp = *(const V *)&phi[base + rowp + x];
+        const V vym = *(const V *)&phi[base + rowm + x];
+        V p = __builtin_nontemporal_load((const V *)&pi[base + row + x]);
+#pragma unroll
+        for (int jj = 0; jj < W; jj++) {
+          const T right = (jj == W - 1) ? vxp : fc[(jj + 1) % W];
+          const T left = (jj == 0) ? vxm : fc[(jj + W - 1) % W];
+          T pj = p[jj];
+          if constexpr (sizeof(T) == 8) {
+            CoefF64 k{d.a2, d.inv_a2, d.c2, d.lmode, d.kappa};
+            if (d.pow2) site_update_f64<true>(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+            else site_update_f64<false>(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+          } else {
+            CoefF32 k{d.f_inv_a2, d.f_c2, d.f_c4, d.f_c6};
+            site_update_f32(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+          }
+          p[jj] = pj;
+        }
+        __builtin_nontemporal_store(p, (V *)&pi[base + row + x]);
+        V fnew;
+#pragma unroll
+        for (int jj = 0; jj < W; jj++) fnew[jj] = fc[jj] + p[jj] * dd;
+        __builtin_nontemporal_store(fnew, (V *)&phin[base + row + x]);
+        fm = fc;
+        fc = fp;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static bool pow2d(double v) {
+  if (!(v > 0) || !std::isfinite(v)) return false;
+  int e;
+  return std::frexp(v, &e) == 0.5;
+}
+
+static int vec_width(const pn_batch *b) { return b->precision == PN_FP64 ? 2 : 4; }
+
+static std::pair<int, int> default_tiles(int N) { return {4, N >= 256 ? 16 : 32}; }
+
+// rebuild descriptors and the tile list; mask (optional) selects the
+// realizations that get tiles
+static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
+  const int W = vec_width(b);
+  b->hdesc.resize(b->R);
+  b->htile.assign(b->R + 1, 0);
+  int tiles = 0;
+  for (int r = 0; r < b->R; r++) {
+    const pn_realization &q = b->re[r];
+    BatchDesc &d = b->hdesc[r];
+    auto it = b->tiles.find(q.N);
+    const std::pair<int, int> ts = it != b->tiles.end() ? it->second : default_tiles(q.N);
+    const int decayed = d.decayed;
+    d = BatchDesc();
+    d.decayed = decayed;
+    d.off = b->off[r];
+    d.N = q.N;
+    d.TY = ts.first;
+    d.ZB = ts.second;
+    d.XT = (q.N + 64 * W - 1) / (64 * W);
+    d.YT = (q.N + d.TY - 1) / d.TY;
+    d.ZT = (q.N + d.ZB - 1) / d.ZB;
+    d.a2 = q.a * q.a;
+    d.inv_a2 = 1.0 / d.a2;
+    d.pow2 = pow2d(d.a2);
+    d.c2 = q.mass_mode * q.m2;
+    d.lmode = (double)q.lambda_mode;
+    d.kappa = q.kappa;
+    d.f_inv_a2 = (float)(1.0 / d.a2);
+    d.f_c2 = (float)d.c2;
+    d.f_c4 = (float)(d.lmode / 6.0);
+    d.f_c6 = (float)(1.0 / (120.0 * q.kappa * q.kappa));
+    d.tile_start = tiles;
+    b->htile[r] = tiles;
+    if (!mask || (*mask)[r]) tiles += d.XT * d.YT * d.ZT;
+  }
+  b->htile[b->R] = tiles;
+  b->total_tiles = tiles;
+  PB_HIP(b, hipMemcpyAsync(b->ddesc, b->hdesc.data(), sizeof(BatchDesc) * b->R, hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(b->dtile, b->htile.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  return PN_OK;
+}
+
+static void drop_graphs(pn_batch *b) {
+  for (int i = 0; i < 2; i++)
+    if (b->gexec[i]) {
+      (void)hipGraphExecDestroy(b->gexec[i]);
+      b->gexec[i] = nullptr;
+    }
+}
+
+static void sync_views(pn_batch *b) {
+  for (pn_ctx *v : b->views) v->cur = b->cur;
+}
+
+extern "C" int pn_batch_destroy(pn_batch *b) {
+  if (!b) return PN_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graphs(b);
+  for (pn_ctx *v : b->views) pn_destroy(v);
+  for (int i = 0; i < 2; i++) (void)hipFree(b->phi[i]);
+  (void)hipFree(b->pi);
+  (void)hipFree(b->ddesc);
+  (void)hipFree(b->dtile);
+  (void)hipFree(b->dtk);
+  (void)hipFree(b->dtd);
+  (void)hipFree(b->dbase);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return PN_OK;
+}
+
+extern "C" const char *pn_batch_last_error(const pn_batch *b) {
+  return b ? b->err.c_str() : pn_last_error(nullptr);
+}
+
+extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision, int device,
+                               pn_batch **out) {
+  if (!out || !reals || R < 1) return bfail(nullptr, PN_ERR_ARG, "pn_batch_create: bad argument");
+  *out = nullptr;
+  if (precision != PN_FP64 && precision != PN_FP32)
+    return bfail(nullptr, PN_ERR_ARG, "pn_batch_create: unknown precision");
+  pn_batch *b = new (std::nothrow) pn_batch();
+  if (!b) return bfail(nullptr, PN_ERR_NOMEM, "pn_batch_create: out of host memory");
+  b->R = R;
+  b->precision = precision;
+  b->device = device;
+  b->esize = precision == PN_FP64 ? 8 : 4;
+  b->re.assign(reals, reals + R);
+  const int W = vec_width(b);
+  long long total = 0;
+  for (int r = 0; r < R; r++) {
+    const pn_realization &q = b->re[r];
+    if (q.N < 4 || q.N > 2048 || q.N % W != 0 || !(q.a > 0) || !(q.kappa != 0)) {
+      pn_batch_destroy(b);
+      return bfail(nullptr, PN_ERR_ARG, "pn_batch_create: realization has N < 4, N % vector width != 0, "
+                                        "a <= 0 or Kappa == 0");
+    }
+    b->off.push_back(total);
+    total += (long long)q.N * q.N * q.N;
+    total = (total + 63) / 64 * 64;  // keep every realization 256/512-byte aligned
+  }
+  b->total = total;
+  auto fail = [&](int code, const char *m, const char *x = "") {
+    int rc = bfail(nullptr, code, m, x);
+    pn_batch_destroy(b);
+    return rc;
+  };
+  if (hipSetDevice(device) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(PN_ERR_HIP, "pn_batch_create: stream creation failed");
+  const size_t bytes = (size_t)total * b->esize;
+  for (int i = 0; i < 2; i++)
+    if (hipMalloc(&b->phi[i], bytes) != hipSuccess) return fail(PN_ERR_NOMEM, "pn_batch_create: field allocation failed");
+  if (hipMalloc(&b->pi, bytes) != hipSuccess) return fail(PN_ERR_NOMEM, "pn_batch_create: field allocation failed");
+  if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * R) != hipSuccess || hipMalloc(&b->dtile, sizeof(int) * (R + 1)) != hipSuccess ||
+      hipMalloc(&b->dbase, sizeof(int)) != hipSuccess)
+    return fail(PN_ERR_NOMEM, "pn_batch_create: descriptor allocation failed");
+  for (int i = 0; i < 2; i++)
+    if (hipMemsetAsync(b->phi[i], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
+  if (hipMemsetAsync(b->pi, 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
+  b->hdesc.assign(R, BatchDesc());
+  if (upload_desc(b) != PN_OK) return fail(PN_ERR_HIP, "pn_batch_create: descriptor upload failed");
+  for (int r = 0; r < R; r++) {
+    const pn_realization &q = b->re[r];
+    pn_params p{q.N, q.a, q.mass_mode, q.m2, q.lambda_mode, q.kappa, precision, device};
+    char *p0 = (char *)b->phi[0] + b->off[r] * b->esize;
+    char *p1 = (char *)b->phi[1] + b->off[r] * b->esize;
+    char *pp = (char *)b->pi + b->off[r] * b->esize;
+    pn_ctx *v = nullptr;
+    if (pn_make_view(&p, p0, p1, pp, 0, b->stream, &v) != PN_OK)
+      return fail(PN_ERR_NOMEM, "pn_batch_create: view creation failed");
+    b->views.push_back(v);
+  }
+  *out = b;
+  return PN_OK;
+}
+
+extern "C" int pn_batch_view(pn_batch *b, int r, pn_ctx **view) {
+  if (!b || !view || r < 0 || r >= b->R) return bfail(b, PN_ERR_ARG, "pn_batch_view: bad argument");
+  b->views[r]->cur = b->cur;
+  *view = b->views[r];
+  return PN_OK;
+}
+
+extern "C" int pn_batch_set_decayed(pn_batch *b, const int *flags) {
+  if (!b || !flags) return bfail(b, PN_ERR_ARG, "pn_batch_set_decayed: NULL argument");
+  for (int r = 0; r < b->R; r++) b->hdesc[r].decayed = flags[r] ? 1 : 0;
+  PB_HIP(b, hipSetDevice(b->device));
+  PB_HIP(b, hipMemcpyAsync(b->ddesc, b->hdesc.data(), sizeof(BatchDesc) * b->R, hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  return PN_OK;
+}
+
+extern "C" int pn_batch_set_tiles(pn_batch *b, int N, int rows, int zblock) {
+  if (!b || rows < 1 || rows > 64 || zblock < 1 || zblock > 2048)
+    return bfail(b, PN_ERR_ARG, "pn_batch_set_tiles: bad argument");
+  b->tiles[N] = {rows, zblock};
+  drop_graphs(b);
+  return upload_desc(b);
+}
+
+static int grid_blocks(const pn_batch *b) {
+  return std::max(1, std::min(b->total_tiles, 256 * 8));
+}
+
+static void launch_batch_step(pn_batch *b, int parity, int j) {
+  const int W = vec_width(b);
+  const int grid = grid_blocks(b);
+  if (b->precision == PN_FP64)
+    hipLaunchKernelGGL((k_batch_step<double, 2>), dim3(grid), dim3(64, 4), 0, b->stream,
+                       (const double *)b->phi[parity], (double *)b->pi, (double *)b->phi[parity ^ 1],
+                       b->ddesc, b->dtile, b->R, b->total_tiles, b->dtk, b->dtd, b->table_cap,
+                       b->dbase, j);
+  else
+    hipLaunchKernelGGL((k_batch_step<float, 4>), dim3(grid), dim3(64, 4), 0, b->stream,
+                       (const float *)b->phi[parity], (float *)b->pi, (float *)b->phi[parity ^ 1],
+                       b->ddesc, b->dtile, b->R, b->total_tiles, b->dtk, b->dtd, b->table_cap,
+                       b->dbase, j);
+  (void)W;
+}
+
+static constexpr int kGraphLen = 100;  // steps per captured graph (even: parity preserved)
+
+static int ensure_graph(pn_batch *b, int parity) {
+  if (b->gexec[parity] && b->graph_len == kGraphLen) return PN_OK;
+  hipGraph_t g = nullptr;
+  PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int j = 0; j < kGraphLen; j++) launch_batch_step(b, parity ^ (j & 1), j);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_evolve: graph capture: ", hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->gexec[parity], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_evolve: graph instantiate: ", hipGetErrorString(e));
+  b->graph_len = kGraphLen;
+  return PN_OK;
+}
+
+static int upload_tables(pn_batch *b, int nsteps, const double *dtk, const double *dtd) {
+  if (nsteps > b->table_cap) {
+    PB_HIP(b, hipStreamSynchronize(b->stream));
+    (void)hipFree(b->dtk);
+    (void)hipFree(b->dtd);
+    b->dtk = b->dtd = nullptr;
+    b->table_cap = 0;
+    const int cap = std::max(nsteps, 256);
+    PB_HIP(b, hipMalloc(&b->dtk, sizeof(double) * (size_t)b->R * cap));
+    PB_HIP(b, hipMalloc(&b->dtd, sizeof(double) * (size_t)b->R * cap));
+    b->table_cap = cap;
+    drop_graphs(b);  // kernel arguments changed
+  }
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  for (int r = 0; r < b->R; r++) {
+    PB_HIP(b, hipMemcpyAsync(b->dtk + (size_t)r * b->table_cap, dtk + (size_t)r * nsteps,
+                             sizeof(double) * nsteps, hipMemcpyHostToDevice, b->stream));
+    PB_HIP(b, hipMemcpyAsync(b->dtd + (size_t)r * b->table_cap, dtd + (size_t)r * nsteps,
+                             sizeof(double) * nsteps, hipMemcpyHostToDevice, b->stream));
+  }
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  return PN_OK;
+}
+
+// enqueues nsteps steps that read the uploaded tables
+static int run_steps(pn_batch *b, int nsteps, int use_graph) {
+  int s = 0;
+  while (s < nsteps) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->dbase, s);
+    if (use_graph && nsteps - s >= kGraphLen) {
+      int rc = ensure_graph(b, b->cur);
+      if (rc != PN_OK) return rc;
+      PB_HIP(b, hipGraphLaunch(b->gexec[b->cur], b->stream));
+      s += kGraphLen;  // even length: parity unchanged
+    } else {
+      const int n = nsteps - s;
+      for (int j = 0; j < n; j++) {
+        launch_batch_step(b, b->cur, j);
+        b->cur ^= 1;
+      }
+      s += n;
+    }
+    PB_HIP(b, hipGetLastError());
+  }
+  sync_views(b);
+  return PN_OK;
+}
+
+// dtk/dtd: [R][nsteps] host tables (per-realization time replay)
+extern "C" int pn_batch_evolve(pn_batch *b, int nsteps, const double *dtk, const double *dtd,
+                               int use_graph) {
+  if (!b || nsteps < 0 || (nsteps > 0 && (!dtk || !dtd)))
+    return bfail(b, PN_ERR_ARG, "pn_batch_evolve: bad argument");
+  if (nsteps == 0) return PN_OK;
+  PB_HIP(b, hipSetDevice(b->device));
+  int rc = upload_tables(b, nsteps, dtk, dtd);
+  if (rc != PN_OK) return rc;
+  return run_steps(b, nsteps, use_graph);
+}
+
+extern "C" int pn_batch_sync(pn_batch *b) {
+  if (!b) return bfail(b, PN_ERR_ARG, "pn_batch_sync: NULL batch");
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  return PN_OK;
+}
+
+extern "C" int pn_batch_evolve_timed(pn_batch *b, int nsteps, const double *dtk, const double *dtd,
+                                     int use_graph, double *ms) {
+  if (!b || !ms) return bfail(b, PN_ERR_ARG, "pn_batch_evolve_timed: NULL argument");
+  if (nsteps < 1 || !dtk || !dtd) return bfail(b, PN_ERR_ARG, "pn_batch_evolve_timed: bad argument");
+  PB_HIP(b, hipSetDevice(b->device));
+  // tables are uploaded (and graphs captured) before the timed region
+  int rc = upload_tables(b, nsteps, dtk, dtd);
+  if (rc != PN_OK) return rc;
+  if (use_graph && nsteps >= kGraphLen && (rc = ensure_graph(b, b->cur)) != PN_OK) return rc;
+  hipEvent_t e0, e1;
+  PB_HIP(b, hipEventCreate(&e0));
+  PB_HIP(b, hipEventCreate(&e1));
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  PB_HIP(b, hipEventRecord(e0, b->stream));
+  rc = run_steps(b, nsteps, use_graph);
+  if (rc == PN_OK) {
+    PB_HIP(b, hipEventRecord(e1, b->stream));
+    PB_HIP(b, hipEventSynchronize(e1));
+    float f = 0;
+    PB_HIP(b, hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// Autotuner: for every size class, time each candidate tile shape on that
+// class's realizations only, keep the fastest, then restore the batch.
+extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nclasses) {
+  if (!b || trial_steps < 1) return bfail(b, PN_ERR_ARG, "pn_batch_autotune: bad argument");
+  PB_HIP(b, hipSetDevice(b->device));
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  const size_t bytes = (size_t)b->total * b->esize;
+  void *save_phi = nullptr, *save_pi = nullptr;
+  PB_HIP(b, hipMalloc(&save_phi, bytes));
+  hipError_t ea = hipMalloc(&save_pi, bytes);
+  if (ea != hipSuccess) {
+    (void)hipFree(save_phi);
+    return bfail(b, PN_ERR_NOMEM, "pn_batch_autotune: no room for the state backup");
+  }
+  const int saved_cur = b->cur;
+  PB_HIP(b, hipMemcpyAsync(save_phi, b->phi[saved_cur], bytes, hipMemcpyDeviceToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(save_pi, b->pi, bytes, hipMemcpyDeviceToDevice, b->stream));
+  std::vector<double> tk((size_t)b->R * trial_steps, 1e-3), td((size_t)b->R * trial_steps, 1e-3);
+  std::vector<int> classes;
+  for (auto &q : b->re)
+    if (std::find(classes.begin(), classes.end(), q.N) == classes.end()) classes.push_back(q.N);
+  std::sort(classes.begin(), classes.end());
+  const int rows_c[] = {2, 4, 8, 16};
+  const int zb_c[] = {8, 16, 32, 64};
+  int rc = PN_OK;
+  std::vector<int> results;
+  for (int N : classes) {
+    std::vector<int> mask(b->R, 0);
+    for (int r = 0; r < b->R; r++) mask[r] = b->re[r].N == N;
+    double best = 1e30;
+    std::pair<int, int> bestp = default_tiles(N);
+    for (int ty : rows_c)
+      for (int zb : zb_c) {
+        if (ty > N || zb > N) continue;
+        b->tiles[N] = {ty, zb};
+        if ((rc = upload_desc(b, &mask)) != PN_OK) break;
+        double ms = 0;
+        if ((rc = pn_batch_evolve(b, trial_steps, tk.data(), td.data(), 0)) != PN_OK) break;  // warm
+        if ((rc = pn_batch_evolve_timed(b, trial_steps, tk.data(), td.data(), 0, &ms)) != PN_OK) break;
+        if (ms < best) {
+          best = ms;
+          bestp = {ty, zb};
+        }
+      }
+    if (rc != PN_OK) break;
+    b->tiles[N] = bestp;
+    results.push_back(N);
+    results.push_back(bestp.first);
+    results.push_back(bestp.second);
+  }
+  // restore the state and the full tile list
+  hipError_t e = hipStreamSynchronize(b->stream);
+  b->cur = saved_cur;
+  if (e == hipSuccess) e = hipMemcpyAsync(b->phi[saved_cur], save_phi, bytes, hipMemcpyDeviceToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->pi, save_pi, bytes, hipMemcpyDeviceToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(save_phi);
+  (void)hipFree(save_pi);
+  drop_graphs(b);
+  sync_views(b);
+  if (rc != PN_OK) return rc;
+  if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_autotune: restore: ", hipGetErrorString(e));
+  if ((rc = upload_desc(b)) != PN_OK) return rc;
+  if (nclasses) *nclasses = (int)classes.size();
+  if (out) std::copy(results.begin(), results.end(), out);
+  return PN_OK;
+}

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "pn_internal.h"
+#include "pn_kernels.h"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -50,16 +51,6 @@ extern "C" int pn_abi_version(void) { return PN_ABI_VERSION; }
 // ---------------------------------------------------------------------------
 // launch geometry
 // ---------------------------------------------------------------------------
-// Blocks are dispatched round-robin over the 8 XCDs of an MI355X.  The remap
-// gives XCD k a contiguous run of logical tiles, so tiles that share y and z
-// halo rows are processed on the same XCD and find each other's planes in
-// that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
-  const int q = nblocks >> 3, r = nblocks & 7;
-  const int k = b & 7, j = b >> 3;
-  return k * q + (k < r ? k : r) + j;
-}
-
 struct StepGeom {
   int N, XT, YT, ZT;
 };
@@ -76,46 +67,6 @@ static inline StepGeom step_geom(int N, int by, int zb) {
 // ---------------------------------------------------------------------------
 // K1: fused kick + drift (updateMomenta, updateFields, exchange)
 // ---------------------------------------------------------------------------
-struct CoefF64 {
-  double a2, inv_a2, c2, lmode, kappa;
-};
-struct CoefF32 {
-  float inv_a2, c2, c4, c6;
-};
-
-// Theory-0 update of one site, reference order (dynamics.cpp:80,84,183).
-template <bool POW2>
-__device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, double xm, double yp,
-                                                double ym, double zp, double zm, double f,
-                                                double &p, double dtk) {
-  double s = xp + xm;
-  s = s + yp;
-  s = s + ym;
-  s = s + zp;
-  s = s + zm;
-  double lap = POW2 ? (s - 6. * f) * k.inv_a2 : (s - 6. * f) / k.a2;
-  double q = 0.0 + f * f;
-  double B = k.lmode * (q / 6.);
-  double t = q / k.kappa;
-  double C = (t * t) / 120.;
-  p = p + (lap - ((k.c2 + B) + C) * f) * dtk;
-}
-
-// FP32 form: same stencil, potential with host-rounded coefficients.
-__device__ __forceinline__ void site_update_f32(const CoefF32 &k, float xp, float xm, float yp,
-                                                float ym, float zp, float zm, float f, float &p,
-                                                float dtk) {
-  float s = xp + xm;
-  s = s + yp;
-  s = s + ym;
-  s = s + zp;
-  s = s + zm;
-  float lap = (s - 6.0f * f) * k.inv_a2;
-  float q = f * f;
-  float force = ((k.c2 + k.c4 * q) + k.c6 * (q * q)) * f;
-  p = p + (lap - force) * dtk;
-}
-
 // One thread per (x,y) column of a 64 x BY tile, streaming ZB planes in z and
 // carrying phi(z-1), phi(z), phi(z+1) in registers.  In-plane neighbours are
 // read from the same or adjacent cache lines (x) or from rows the tile's other
@@ -173,11 +124,6 @@ __global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *
 // loads and stores: W = 2 doubles or 4 floats), a wave covers 64*W sites of
 // the row.  Neighbours inside the vector come from registers; only the two
 // outer x neighbours are separate loads.  Needs N % W == 0.
-template <typename T, int W>
-struct VecT {
-  typedef T type __attribute__((ext_vector_type(W)));
-};
-
 template <typename T, typename C, bool POW2, int W, int BY, int ZB, int MODE>
 __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi, T *__restrict__ pi,
                                                       T *__restrict__ phin, StepGeom g, C k,
@@ -394,6 +340,37 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
   return PN_OK;
 }
 
+int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, hipStream_t s,
+                 pn_ctx **out) {
+  pn_ctx *c = new (std::nothrow) pn_ctx();
+  if (!c) return pn_fail(nullptr, PN_ERR_NOMEM, "pn_make_view: out of host memory");
+  c->p = *p;
+  c->is_view = 1;
+  c->n3 = (long long)p->N * p->N * p->N;
+  c->esize = p->precision == PN_FP64 ? 8 : 4;
+  c->k.a2 = p->a * p->a;
+  c->k.a2_pow2 = is_pow2_double(c->k.a2);
+  c->k.inv_a2 = 1.0 / c->k.a2;
+  c->k.c2 = p->mass_mode * p->m2;
+  c->k.lmode = (double)p->lambda_mode;
+  c->k.kappa = p->kappa;
+  c->red_blocks = (int)std::min<long long>(1024, (c->n3 + kRedThreads - 1) / kRedThreads);
+  c->phi[0] = phi0;
+  c->phi[1] = phi1;
+  c->pi = pi;
+  c->cur = cur;
+  c->stream = s;
+  if (hipMalloc(&c->red_part, (size_t)c->red_blocks * 2 * kNSums * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->ext_part, (size_t)c->red_blocks * 6 * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->red_out, 64 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->host_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    pn_destroy(c);
+    return pn_fail(nullptr, PN_ERR_NOMEM, "pn_make_view: cannot allocate reduction scratch");
+  }
+  *out = c;
+  return PN_OK;
+}
+
 extern "C" int pn_set_couplings(pn_ctx *c, int mass_mode, double m2, int lambda_mode, double kappa) {
   if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_set_couplings: NULL context");
   if (!(kappa != 0) || !std::isfinite(kappa))
@@ -414,14 +391,16 @@ extern "C" int pn_destroy(pn_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   pn_free_spectrum(c);
   pn_free_islands(c);
-  for (int i = 0; i < 2; i++)
-    if (c->phi[i]) hipFree(c->phi[i]);
-  if (c->pi) hipFree(c->pi);
-  if (c->red_part) hipFree(c->red_part);
-  if (c->ext_part) hipFree(c->ext_part);
-  if (c->red_out) hipFree(c->red_out);
-  if (c->host_pin) hipHostFree(c->host_pin);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (!c->is_view) {
+    for (int i = 0; i < 2; i++)
+      if (c->phi[i]) (void)hipFree(c->phi[i]);
+    if (c->pi) (void)hipFree(c->pi);
+  }
+  if (c->red_part) (void)hipFree(c->red_part);
+  if (c->ext_part) (void)hipFree(c->ext_part);
+  if (c->red_out) (void)hipFree(c->red_out);
+  if (c->host_pin) (void)hipHostFree(c->host_pin);
+  if (c->stream && !c->is_view) (void)hipStreamDestroy(c->stream);
   delete c;
   return PN_OK;
 }
@@ -492,6 +471,7 @@ extern "C" int pn_get_fields(pn_ctx *c, double *phi, double *pi, int layout) {
 // ---------------------------------------------------------------------------
 extern "C" int pn_kick(pn_ctx *c, double dtk) {
   if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_kick: NULL context");
+  if (c->is_view) return pn_fail(c, PN_ERR_STATE, "pn_kick: a batch view is stepped by pn_batch_evolve");
   PN_HIP(c, hipSetDevice(c->p.device));
   int rc = launch_step(c, 1, dtk, 0.0);
   if (rc != PN_OK) return rc;
@@ -501,6 +481,7 @@ extern "C" int pn_kick(pn_ctx *c, double dtk) {
 
 extern "C" int pn_drift(pn_ctx *c, double dtd) {
   if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_drift: NULL context");
+  if (c->is_view) return pn_fail(c, PN_ERR_STATE, "pn_drift: a batch view is stepped by pn_batch_evolve");
   PN_HIP(c, hipSetDevice(c->p.device));
   const int threads = 256;
   const int blocks = (int)std::min<long long>(4096, (c->n3 + threads - 1) / threads);
@@ -519,6 +500,7 @@ extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double 
   if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_evolve: NULL context");
   if (nsteps < 0 || (nsteps > 0 && (!dtk || !dtd)))
     return pn_fail(c, PN_ERR_ARG, "pn_evolve: bad step arguments");
+  if (c->is_view) return pn_fail(c, PN_ERR_STATE, "pn_evolve: a batch view is stepped by pn_batch_evolve");
   PN_HIP(c, hipSetDevice(c->p.device));
   for (int n = 0; n < nsteps; n++) {
     int rc = launch_step(c, 0, dtk[n], dtd[n]);
@@ -561,26 +543,6 @@ extern "C" int pn_sync(pn_ctx *c) {
 // Sums are FP64 and compensated: each thread keeps (hi, lo) pairs updated
 // with an exact two-sum, partial pairs are merged in a fixed tree, so the
 // result is deterministic and within a few ulp of the exact sum.
-__device__ __forceinline__ void two_acc(double &hi, double &lo, double x) {
-  const double s = hi + x;
-  const double bp = s - hi;
-  const double e = (hi - (s - bp)) + (x - bp);
-  hi = s;
-  lo = lo + e;
-}
-
-__device__ __forceinline__ void two_merge(double &h1, double &l1, double h2, double l2) {
-  two_acc(h1, l1, h2);
-  l1 = l1 + l2;
-}
-
-struct RedCoef {
-  double a2;    // sqr(a_t)
-  double cm;    // (Mass_Mode*0.5)*BareM_sqr_long        (fieldanalysis.cpp:457)
-  double lmode; // Lambda_Mode                           (:571)
-  double d6;    // 720.0*Nc*Nc*Kappa*Kappa               (:571)
-};
-
 // extrema candidates: strict comparisons in raster order, ties -> lowest index
 struct Ext {
   double mn, mx, ab;

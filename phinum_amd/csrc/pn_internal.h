@@ -41,7 +41,14 @@ struct pn_ctx {
   pn_fft_state *fft;
   pn_island_state *isl;
   std::string err;
+  // a view aliases one realization of a batch: it owns neither the field
+  // buffers nor the stream, and cannot step on its own (the batch steps it)
+  int is_view;
 };
+
+// builds a view context over existing device buffers (pn_batch.hip)
+int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, hipStream_t s,
+                 pn_ctx **out);
 
 // error helpers ------------------------------------------------------------
 int pn_fail(pn_ctx *c, int code, const char *fmt, ...);
