@@ -57,6 +57,45 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m):
     return total, np.array(energies)
 
 
+def batch_leg(pn, steps, out_every=500):
+    """64 realizations: 32 x 64^3, 24 x 128^3, 8 x 192^3, seeds 0..63, lambda = 1,
+    g drawn uniformly from [0.1, 0.4] with numpy's default_rng(0); FP64, one
+    launch per step for all of them, 100-step HIP graphs, tiles autotuned per
+    size class.  Energies and the false-vacuum check every `out_every` steps
+    (outside the timing).  Returns a dict for the JSON line."""
+    sizes = [64] * 32 + [128] * 24 + [192] * 8
+    g = np.random.default_rng(0).uniform(0.1, 0.4, len(sizes))
+    reals = [dict(N=N, a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1,
+                  kappa=float(np.sqrt(36.0 / (120.0 * gg)))) for N, gg in zip(sizes, g)]
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, steps)
+    with pn.Batch(reals) as B:
+        for r in range(len(sizes)):
+            B.view(r).initial_conditions(pn.ic_params(r, a=0.5, Lambda=6.0, phi0=float(np.sqrt(6.0))))
+        tiles = B.autotune(trial_steps=5)
+        e0 = np.array([B.view(r).volume_averages().energy for r in range(len(sizes))])
+        decayed = np.zeros(len(sizes), dtype=np.int32)
+        total = 0.0
+        s = 0
+        while s < steps:
+            n = min(out_every, steps - s)
+            total += B.evolve_timed(dtk[s:s + n], dtd[s:s + n], use_graph=True)
+            s += n
+            av = [B.view(r).volume_averages() for r in range(len(sizes))]
+            # false_vacuum_decayed (fieldanalysis.cpp:887-893) per realization
+            newly = np.array([(a.interaction < 0 or a.interaction != a.interaction) for a in av])
+            if np.any(newly & (decayed == 0)):
+                decayed |= newly.astype(np.int32)
+                B.set_decayed(decayed)
+        e1 = np.array([a.energy for a in av])
+    sites = float(sum(N ** 3 for N in sizes))
+    return {"value": sites * steps / (total * 1e-3), "unit": "site-updates/s",
+            "realizations": len(sizes), "sites": int(sites), "steps": steps,
+            "ms_per_step": total / steps, "tiles_rows_zblock": tiles,
+            "roofline_frac": 32.0 * sites / (total / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "decayed": int(decayed.sum()),
+            "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0)))}
+
+
 def cpu_baseline(N, m, target_s=12.0):
     import ctypes as C
     path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
@@ -87,6 +126,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--out-every", type=int, default=200)
+    ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -152,6 +192,8 @@ def main():
         line["roofline_512"] = {"bound": "hbm", "achieved": a5, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": a5 / HBM_PEAK_GBS, "ms_per_step": ms5 / 200,
                                 "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3)}
+        log("batch of 64 realizations")
+        line["batch_64"] = batch_leg(pn, args.batch_steps)
     if not args.no_cpu:
         log("cpu baseline")
         cb = cpu_baseline(N, m)

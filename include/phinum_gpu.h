@@ -243,6 +243,43 @@ int pn_batch_set_tiles(pn_batch *b, int N, int rows, int zblock);
  * (N, rows, zblock); pass NULL to skip. */
 int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nclasses);
 
+/* ---- differentiable leapfrog (no reference counterpart) ----------------- */
+
+/* The theory-0 leapfrog of dynamics.cpp:52-237 with the force as a
+ * polynomial with continuous couplings: f = ((c2 + c4 q) + c6 q^2) phi,
+ * q = phi^2 (reference units: c2 = Mass_Mode*M^2, c4 = Lambda_Mode/6,
+ * c6 = 1/(120 Kappa^2); physical units of V = phi^2/2 - lambda phi^4/4 +
+ * g phi^6/6: c2 = 1, c4 = -lambda, c6 = g).  Observable after the last step:
+ * F = sum rho s(u) / sum rho, u = (rho - threshold*rhobar)/(width*rhobar),
+ * s logistic, rho the per-site summand of Energy (fieldanalysis.cpp:625). */
+typedef struct pn_adj_params {
+  int N;
+  double a;
+  double c2, c4, c6;
+  double threshold;      /* 4   */
+  double width;          /* 0.1 */
+  int checkpoint_every;  /* K: (phi, pi) saved every K steps */
+  int device;
+} pn_adj_params;
+
+typedef struct pn_adj_result {
+  double F;
+  double grad_c[3];      /* dF/dc2, dF/dc4, dF/dc6 */
+  double forward_ms;     /* device time of the checkpointed forward + F */
+  double backward_ms;    /* device time of recompute + reverse sweep   */
+  long long peak_bytes;  /* device memory held by the run              */
+} pn_adj_result;
+
+/* F after nsteps steps from (phi0, pi0), dense N^3 host arrays. */
+int pn_forward_observable(const pn_adj_params *p, const double *phi0, const double *pi0, int nsteps,
+                          const double *dtk, const double *dtd, double *F);
+/* Reverse entry point: F, dF/dphi0 and dF/dpi0 (N^3 each, either may be
+ * NULL) and dF/dc in res, by checkpointed recompute (every
+ * checkpoint_every steps) and a hand-written adjoint stencil. */
+int pn_adjoint(const pn_adj_params *p, const double *phi0, const double *pi0, int nsteps,
+               const double *dtk, const double *dtd, double *grad_phi0, double *grad_pi0,
+               pn_adj_result *res);
+
 #ifdef __cplusplus
 }
 #endif

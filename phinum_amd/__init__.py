@@ -69,6 +69,17 @@ class PnRealization(C.Structure):
                 ("lambda_mode", C.c_int), ("kappa", C.c_double)]
 
 
+class PnAdjParams(C.Structure):
+    _fields_ = [("N", C.c_int), ("a", C.c_double), ("c2", C.c_double), ("c4", C.c_double),
+                ("c6", C.c_double), ("threshold", C.c_double), ("width", C.c_double),
+                ("checkpoint_every", C.c_int), ("device", C.c_int)]
+
+
+class PnAdjResult(C.Structure):
+    _fields_ = [("F", C.c_double), ("grad_c", C.c_double * 3), ("forward_ms", C.c_double),
+                ("backward_ms", C.c_double), ("peak_bytes", C.c_longlong)]
+
+
 class PnCatalog(C.Structure):
     _fields_ = [("capacity", C.c_int), ("count", C.c_int), ("threshold", C.c_double),
                 ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
@@ -120,6 +131,8 @@ def lib():
         "pn_batch_set_decayed": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "pn_batch_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "pn_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]),
+        "pn_forward_observable": (C.c_int, [P(PnAdjParams), dp, dp, C.c_int, dp, dp, dp]),
+        "pn_adjoint": (C.c_int, [P(PnAdjParams), dp, dp, C.c_int, dp, dp, dp, dp, P(PnAdjResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -420,3 +433,40 @@ class Batch:
         self._chk(self.L.pn_batch_autotune(self.h, int(trial_steps),
                                            out.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n)))
         return [tuple(int(v) for v in out[3 * i:3 * i + 3]) for i in range(n.value)]
+
+
+def adj_params(N, c2=1.0, c4=-1.0, c6=0.2, a=0.5, threshold=4.0, width=0.1, checkpoint_every=100,
+               device=0):
+    """Couplings of the differentiable leapfrog: V = c2 phi^2/2 + c4 phi^4/4 + c6 phi^6/6
+    (physical units of V = phi^2/2 - lambda phi^4/4 + g phi^6/6: c4 = -lambda, c6 = g)."""
+    return PnAdjParams(int(N), float(a), float(c2), float(c4), float(c6), float(threshold),
+                       float(width), int(checkpoint_every), int(device))
+
+
+def forward_observable(p, phi0, pi0, dtk, dtd):
+    phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+    pi0 = np.ascontiguousarray(pi0, dtype=np.float64)
+    dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+    dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+    F = C.c_double()
+    rc = lib().pn_forward_observable(C.byref(p), _dptr(phi0), _dptr(pi0), int(dtk.size), _dptr(dtk),
+                                     _dptr(dtd), C.byref(F))
+    if rc != PN_OK:
+        raise PhiNumError(rc, lib().pn_last_error(None).decode())
+    return F.value
+
+
+def adjoint(p, phi0, pi0, dtk, dtd):
+    """Reverse entry point: returns (F, dF/dphi0, dF/dpi0, dF/dc (3,), result struct)."""
+    phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+    pi0 = np.ascontiguousarray(pi0, dtype=np.float64)
+    dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+    dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+    gphi = np.empty_like(phi0)
+    gpi = np.empty_like(pi0)
+    res = PnAdjResult()
+    rc = lib().pn_adjoint(C.byref(p), _dptr(phi0), _dptr(pi0), int(dtk.size), _dptr(dtk), _dptr(dtd),
+                          _dptr(gphi), _dptr(gpi), C.byref(res))
+    if rc != PN_OK:
+        raise PhiNumError(rc, lib().pn_last_error(None).decode())
+    return res.F, gphi, gpi, np.array(list(res.grad_c)), res

@@ -282,3 +282,71 @@ def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtb
     labels = np.frombuffer(b, np.int32, n3, o).copy()
     return dict(phi=phi, pi=pi, avg=avg, ext=ext, extpos=extpos, thr=thr_used, cat=cat,
                 labels=labels)
+
+
+# ---------------------------------------------------------------------------
+# differentiable leapfrog checker (oracle/oracle_adjoint.c)
+# ---------------------------------------------------------------------------
+class OrAdjParams(C.Structure):
+    _fields_ = [("N", C.c_int), ("a2", C.c_double), ("c2", C.c_double), ("c4", C.c_double),
+                ("c6", C.c_double), ("t", C.c_double), ("w", C.c_double)]
+
+
+def _adj_lib():
+    L = lib()
+    if not getattr(L, "_adj_ready", False):
+        dp = C.POINTER(C.c_double)
+        P = C.POINTER(OrAdjParams)
+        for name, res, args in [
+            ("or_poly_evolve", None, [P, dp, dp, C.c_int, dp, dp]),
+            ("or_observable", C.c_double, [P, dp, dp, dp, dp]),
+            ("or_forward_F", C.c_double, [P, dp, dp, C.c_int, dp, dp]),
+            ("or_tlm", C.c_double, [P, dp, dp, C.c_int, dp, dp, dp, dp, dp]),
+            ("or_adjoint", C.c_double, [P, dp, dp, C.c_int, dp, dp, dp, dp, dp]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._adj_ready = True
+    return L
+
+
+def adj_params(N, c2=1.0, c4=-1.0, c6=0.2, a=0.5, t=4.0, w=0.1):
+    """Physical-units theory 0: V = c2 phi^2/2 + c4 phi^4/4 + c6 phi^6/6 (c4 = -lambda, c6 = g)."""
+    return OrAdjParams(N, a * a, c2, c4, c6, t, w)
+
+
+def adj_forward_F(p, phi0, pi0, dtk, dtd):
+    return _adj_lib().or_forward_F(C.byref(p), dptr(np.ascontiguousarray(phi0)),
+                                   dptr(np.ascontiguousarray(pi0)), len(dtk),
+                                   dptr(np.ascontiguousarray(dtk)), dptr(np.ascontiguousarray(dtd)))
+
+
+def adj_observable(p, phi, pi):
+    n3 = p.N ** 3
+    rho = np.empty(n3)
+    g = np.empty(n3)
+    F = _adj_lib().or_observable(C.byref(p), dptr(np.ascontiguousarray(phi)),
+                                 dptr(np.ascontiguousarray(pi)), dptr(rho), dptr(g))
+    return F, rho, g
+
+
+def adj_tlm(p, phi0, pi0, dtk, dtd, dphi0, dpi0, dc):
+    dc = np.ascontiguousarray(dc, dtype=np.float64)
+    return _adj_lib().or_tlm(C.byref(p), dptr(np.ascontiguousarray(phi0)),
+                             dptr(np.ascontiguousarray(pi0)), len(dtk),
+                             dptr(np.ascontiguousarray(dtk)), dptr(np.ascontiguousarray(dtd)),
+                             dptr(np.ascontiguousarray(dphi0)), dptr(np.ascontiguousarray(dpi0)),
+                             dptr(dc))
+
+
+def adj_reverse(p, phi0, pi0, dtk, dtd):
+    n3 = p.N ** 3
+    gphi = np.empty(n3)
+    gpi = np.empty(n3)
+    gc = np.empty(3)
+    F = _adj_lib().or_adjoint(C.byref(p), dptr(np.ascontiguousarray(phi0)),
+                              dptr(np.ascontiguousarray(pi0)), len(dtk),
+                              dptr(np.ascontiguousarray(dtk)), dptr(np.ascontiguousarray(dtd)),
+                              dptr(gphi), dptr(gpi), dptr(gc))
+    return F, gphi, gpi, gc
