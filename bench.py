@@ -96,6 +96,40 @@ def batch_leg(pn, steps, out_every=500):
             "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0)))}
 
 
+def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, eps=1e-6):
+    """Reverse-mode gradient of the smooth oscillon fraction after `steps`
+    steps, in physical units (V = phi^2/2 - lam phi^4/4 + g phi^6/6): seeded
+    vacuum IC divided by sqrt(Lambda), checkpoints every K steps.  Checked
+    against central differences in lambda, g and one random phi0 direction."""
+    m = model(lam, g)
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"]) as L:
+        L.initial_conditions(pn.ic_params(7, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
+        phi, pi = L.get_fields()
+    s = float(np.sqrt(m["Lambda"]))
+    phi, pi = phi / s, pi / s
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    p = pn.adj_params(N, c2=1.0, c4=-lam, c6=g, checkpoint_every=K)
+    F, gphi, _, gc, res = pn.adjoint(p, phi, pi, dtk, dtd)
+    dF_dlam, dF_dg = -gc[1], gc[2]
+
+    def fwd(l_, g_, ph):
+        return pn.forward_observable(pn.adj_params(N, c2=1.0, c4=-l_, c6=g_), ph, pi, dtk, dtd)
+
+    fd_lam = (fwd(lam + eps, g, phi) - fwd(lam - eps, g, phi)) / (2 * eps)
+    fd_g = (fwd(lam, g + eps, phi) - fwd(lam, g - eps, phi)) / (2 * eps)
+    v = np.random.default_rng(3).standard_normal(N ** 3)
+    fd_v = (fwd(lam, g, phi + eps * v) - fwd(lam, g, phi - eps * v)) / (2 * eps)
+    ad_v = float(np.dot(gphi, v))
+    return {"N": N, "steps": steps, "checkpoint_every": K, "F": F,
+            "dF_dlambda": dF_dlam, "dF_dg": dF_dg,
+            "rel_err_fd_lambda": abs(fd_lam - dF_dlam) / abs(dF_dlam),
+            "rel_err_fd_g": abs(fd_g - dF_dg) / abs(dF_dg),
+            "rel_err_fd_phi0_direction": abs(fd_v - ad_v) / abs(ad_v),
+            "forward_ms": res.forward_ms, "backward_ms": res.backward_ms,
+            "backward_over_forward": res.backward_ms / res.forward_ms,
+            "peak_device_bytes": res.peak_bytes}
+
+
 def cpu_baseline(N, m, target_s=12.0):
     import ctypes as C
     path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
@@ -194,6 +228,8 @@ def main():
                                 "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3)}
         log("batch of 64 realizations")
         line["batch_64"] = batch_leg(pn, args.batch_steps)
+        log("adjoint 128^3")
+        line["adjoint_128"] = adjoint_leg(pn)
     if not args.no_cpu:
         log("cpu baseline")
         cb = cpu_baseline(N, m)

@@ -167,6 +167,15 @@ int pn_drift(pn_ctx *c, double dtd);
  * are the host-replayed factors of step n.  Enqueued; pn_sync waits. */
 int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double *dtd);
 int pn_sync(pn_ctx *c);
+
+/* Options (no reference counterpart).
+ * PN_OPT_TWO_STEP: 1 (default where the lattice fits: FP64 N <= 512, FP32
+ * N <= 1024, N a multiple of the vector width) makes pn_evolve advance pairs
+ * of steps in one pass over HBM (temporal blocking); 0 makes every step its
+ * own launch.  Both give bit-identical fields. */
+#define PN_OPT_TWO_STEP 1
+int pn_set_option(pn_ctx *c, int option, int value);
+int pn_get_option(pn_ctx *c, int option, int *value);
 /* pn_evolve bracketed by HIP events on the context's stream; waits for the
  * steps and returns their device time in *ms (measurement helper, no
  * reference counterpart). */

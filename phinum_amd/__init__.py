@@ -22,6 +22,7 @@ PN_ERR_NOMEM = -5
 PN_ERR_CAPACITY = -6
 PN_FP64, PN_FP32 = 0, 1
 PN_LAYOUT_REF, PN_LAYOUT_DENSE = 0, 1
+PN_OPT_TWO_STEP = 1
 
 
 class PnParams(C.Structure):
@@ -131,6 +132,8 @@ def lib():
         "pn_batch_set_decayed": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "pn_batch_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "pn_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]),
+        "pn_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "pn_get_option": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int)]),
         "pn_forward_observable": (C.c_int, [P(PnAdjParams), dp, dp, C.c_int, dp, dp, dp]),
         "pn_adjoint": (C.c_int, [P(PnAdjParams), dp, dp, C.c_int, dp, dp, dp, dp, P(PnAdjResult)]),
     }
@@ -194,6 +197,14 @@ class Lattice:
             if getattr(self, "_owned", True):
                 self.L.pn_destroy(self.h)
             self.h = None
+
+    def two_step(self, on=None):
+        """Gets or sets PN_OPT_TWO_STEP (pairs of steps per pass over HBM)."""
+        if on is not None:
+            self._chk(self.L.pn_set_option(self.h, PN_OPT_TWO_STEP, int(bool(on))))
+        v = C.c_int()
+        self._chk(self.L.pn_get_option(self.h, PN_OPT_TWO_STEP, C.byref(v)))
+        return bool(v.value)
 
     def set_couplings(self, mass_mode, m2, lambda_mode, kappa):
         self._chk(self.L.pn_set_couplings(self.h, int(mass_mode), float(m2), int(lambda_mode),

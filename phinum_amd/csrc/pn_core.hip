@@ -182,6 +182,95 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
   }
 }
 
+// Two leapfrog steps per pass over HBM (temporal blocking).  A block owns full
+// x rows (N/W threads of W sites each) and RT thread rows.  Stage A advances
+// step n on all RT rows, i.e. the block's RT-2 output rows plus one halo row on
+// each side; stage B advances step n+1 on the inner RT-2 rows, reading the
+// intermediate Phi of the current plane from LDS and the planes above and
+// below from registers.  The z loop runs stage A one plane ahead of stage B,
+// and one extra plane at each end of the block's z range.  Halo rows and planes
+// are recomputed by both neighbouring blocks with the same arithmetic, so the
+// result is bit-identical to two single steps.  Pi is read from `pi` and
+// written to `pio` (a second buffer): a neighbouring block's stage A still
+// needs the old Pi of the rows and planes this block writes.
+template <typename T, typename C, bool POW2, int W, int RT>
+__global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                T *__restrict__ pio, T *__restrict__ phin, int N,
+                                                int ZB, C k, T dk0, T dd0, T dk1, T dd1) {
+  typedef typename VecT<T, W>::type V;
+  extern __shared__ unsigned char lds_raw[];
+  T *lds = (T *)lds_raw;  // two planes x RT rows x N
+  constexpr int TY = RT - 2;
+  const int YT = (N + TY - 1) / TY;
+  const int yt = (int)blockIdx.x % YT, zt = (int)blockIdx.x / YT;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int x = W * tx;
+  const int y0 = yt * TY;
+  int y = y0 - 1 + ty;
+  y = y < 0 ? y + N : (y >= N ? y - N : y);
+  const bool inner = ty >= 1 && ty <= TY && (y0 + ty - 1) < N;
+  const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  const int z0 = zt * ZB, z1 = min(z0 + ZB, N);
+  auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
+  V an_m = *(const V *)&phi[wz(z0 - 2) * P + row + x];
+  V an_c = *(const V *)&phi[wz(z0 - 1) * P + row + x];
+  V b_m = an_m, b_c = an_c, pih = an_c;
+  for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
+    const long long base = (long long)wz(za) * P;
+    const V an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
+    const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
+    const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
+    V p = *(const V *)&pi[base + row + x];
+    V f1;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
+      const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
+      T pj = p[j];
+      if constexpr (sizeof(T) == 8)
+        site_update_f64<POW2>(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
+      else
+        site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
+      p[j] = pj;
+      f1[j] = an_c[j] + pj * dd0;
+    }
+    T *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
+    *(V *)&buf[ty * N + x] = f1;
+    __syncthreads();
+    if (za >= z0 + 1 && inner) {
+      const int zb = za - 1;
+      const T *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
+      const T bxp = bb[ty * N + xp], bxm = bb[ty * N + xm];
+      const V byp = *(const V *)&bb[(ty + 1) * N + x], bym = *(const V *)&bb[(ty - 1) * N + x];
+      V q = pih, f2;
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
+        const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
+        T qj = q[j];
+        if constexpr (sizeof(T) == 8)
+          site_update_f64<POW2>(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
+        else
+          site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
+        q[j] = qj;
+        f2[j] = b_c[j] + qj * dd1;
+      }
+      const long long bbase = (long long)zb * P;
+      __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
+      __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
+    }
+    __syncthreads();
+    b_m = b_c;
+    b_c = f1;
+    pih = p;
+    an_m = an_c;
+    an_c = an_p;
+  }
+}
+
 // updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.
 template <typename T>
 __global__ void k_drift(T *__restrict__ phi, const T *__restrict__ pi, long long n, T dtd) {
@@ -263,6 +352,63 @@ static int launch_step(pn_ctx *c, int mode, double dtk, double dtd) {
   return PN_OK;
 }
 
+// Two-step tile shape: W sites per thread, RT thread rows, ZB planes per block.
+// Full rows need N/W <= 128 threads per row; LDS holds 2*RT*N values (<= 64 KB).
+struct Step2Shape {
+  int W, RT, ZB;
+};
+
+static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
+  const int N = c->p.N;
+  const bool dp = c->p.precision == PN_FP64;
+  if (N < 4) return false;
+  if (dp) {
+    if (N <= 256 && N % 2 == 0) *s = {2, 6, 32};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 8, 32};
+    else return false;
+  } else {
+    if (N <= 512 && N % 4 == 0) *s = {4, 8, 32};
+    else if (N <= 1024 && N % 8 == 0) *s = {8, 8, 32};
+    else return false;
+  }
+  if (s->ZB > N) s->ZB = N;
+  const size_t lds = (size_t)2 * s->RT * N * c->esize;
+  return (N / s->W) * s->RT <= 1024 && lds <= 65536;
+}
+
+template <typename T, typename C, bool POW2, int W, int RT>
+static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T dd1) {
+  const int N = c->p.N;
+  const int YT = (N + RT - 3) / (RT - 2), ZT = (N + ZB - 1) / ZB;
+  const size_t lds = (size_t)2 * RT * N * sizeof(T);
+  hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
+                     (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
+                     (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
+}
+
+// steps n and n+1 in one pass; Pi moves to the second buffer
+static int launch_step2(pn_ctx *c, const Step2Shape &s, const double *dtk, const double *dtd) {
+  if (c->p.precision == PN_FP64) {
+    CoefF64 k = coef64(c);
+    if (s.W == 2) {
+      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+      else launch_step2_t<double, CoefF64, false, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+    } else {
+      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 8>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+      else launch_step2_t<double, CoefF64, false, 4, 8>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+    }
+  } else {
+    CoefF32 k = coef32(c);
+    const float a0 = (float)dtk[0], b0 = (float)dtd[0], a1 = (float)dtk[1], b1 = (float)dtd[1];
+    if (s.W == 4) launch_step2_t<float, CoefF32, true, 4, 8>(c, k, s.ZB, a0, b0, a1, b1);
+    else launch_step2_t<float, CoefF32, true, 8, 8>(c, k, s.ZB, a0, b0, a1, b1);
+  }
+  PN_CHECK_LAUNCH(c);
+  c->cur ^= 1;
+  std::swap(c->pi, c->pi_alt);
+  return PN_OK;
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -324,6 +470,12 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
       return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate %zu bytes", bytes));
   if (hipMalloc(&c->pi, bytes) != hipSuccess)
     return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate %zu bytes", bytes));
+  Step2Shape s2;
+  if (step2_shape(c, &s2)) {
+    // the two-step kernel needs a second Pi buffer; without the room, step singly
+    if (hipMalloc(&c->pi_alt, bytes) == hipSuccess) c->step2 = 1;
+    else (void)hipGetLastError();
+  }
   if (hipMalloc(&c->red_part, (size_t)c->red_blocks * 2 * kNSums * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->ext_part, (size_t)c->red_blocks * 6 * sizeof(long long)) != hipSuccess ||
       hipMalloc(&c->red_out, 64 * sizeof(double)) != hipSuccess)
@@ -395,6 +547,7 @@ extern "C" int pn_destroy(pn_ctx *c) {
     for (int i = 0; i < 2; i++)
       if (c->phi[i]) (void)hipFree(c->phi[i]);
     if (c->pi) (void)hipFree(c->pi);
+    if (c->pi_alt) (void)hipFree(c->pi_alt);
   }
   if (c->red_part) (void)hipFree(c->red_part);
   if (c->ext_part) (void)hipFree(c->ext_part);
@@ -502,11 +655,38 @@ extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double 
     return pn_fail(c, PN_ERR_ARG, "pn_evolve: bad step arguments");
   if (c->is_view) return pn_fail(c, PN_ERR_STATE, "pn_evolve: a batch view is stepped by pn_batch_evolve");
   PN_HIP(c, hipSetDevice(c->p.device));
-  for (int n = 0; n < nsteps; n++) {
+  int n = 0;
+  Step2Shape s2;
+  if (c->step2 && c->pi_alt && step2_shape(c, &s2)) {
+    for (; n + 1 < nsteps; n += 2) {
+      int rc = launch_step2(c, s2, dtk + n, dtd + n);
+      if (rc != PN_OK) return rc;
+    }
+  }
+  for (; n < nsteps; n++) {
     int rc = launch_step(c, 0, dtk[n], dtd[n]);
     if (rc != PN_OK) return rc;
   }
   return PN_OK;
+}
+
+extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
+  if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_set_option: NULL context");
+  if (option == PN_OPT_TWO_STEP) {
+    if (value && !c->pi_alt) return pn_fail(c, PN_ERR_STATE, "pn_set_option: no two-step kernel for this lattice");
+    c->step2 = value ? 1 : 0;
+    return PN_OK;
+  }
+  return pn_fail(c, PN_ERR_ARG, "pn_set_option: unknown option %d", option);
+}
+
+extern "C" int pn_get_option(pn_ctx *c, int option, int *value) {
+  if (!c || !value) return pn_fail(c, PN_ERR_ARG, "pn_get_option: NULL argument");
+  if (option == PN_OPT_TWO_STEP) {
+    *value = c->step2;
+    return PN_OK;
+  }
+  return pn_fail(c, PN_ERR_ARG, "pn_get_option: unknown option %d", option);
 }
 
 extern "C" int pn_evolve_timed(pn_ctx *c, int nsteps, const double *dtk, const double *dtd,

@@ -30,6 +30,8 @@ struct pn_ctx {
   size_t esize;        // 8 (FP64) or 4 (FP32)
   void *phi[2];        // ping-pong field buffers, dense x-fastest
   void *pi;            // momenta, updated in place
+  void *pi_alt;        // second Pi buffer for the two-step kernel (or NULL)
+  int step2;           // pn_evolve advances pairs of steps in one pass
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch

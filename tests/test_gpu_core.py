@@ -39,6 +39,72 @@ def test_fused_step_bitwise_fp64(N, steps, seed):
     assert np.array_equal(_bits(gpi), _bits(opi))
 
 
+@pytest.mark.parametrize("N,steps,prec", [(4, 7, pn.PN_FP64), (6, 9, pn.PN_FP64), (10, 5, pn.PN_FP64),
+                                          (16, 11, pn.PN_FP64), (34, 6, pn.PN_FP64), (130, 3, pn.PN_FP64),
+                                          (256, 4, pn.PN_FP64), (260, 2, pn.PN_FP64),
+                                          (8, 9, pn.PN_FP32), (36, 5, pn.PN_FP32), (520, 2, pn.PN_FP32)])
+def test_two_step_kernel_bitwise_equals_single_steps(N, steps, prec):
+    # temporal blocking changes only the evaluation order across sites
+    m = O.model(N)
+    phi, pi = _fields(N, 70 + N)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    out = []
+    for two in (True, False):
+        with _lat(m, prec) as L:
+            if two:
+                assert L.two_step(True)
+            else:
+                L.two_step(False)
+            L.set_fields(phi, pi)
+            L.evolve(dtk, dtd)
+            out.append(L.get_fields())
+    assert np.array_equal(_bits(out[0][0]), _bits(out[1][0]))
+    assert np.array_equal(_bits(out[0][1]), _bits(out[1][1]))
+    if prec == pn.PN_FP64 and N <= 34:
+        ophi, opi = O.evolve(m, phi, pi, steps)
+        assert np.array_equal(_bits(out[0][0]), _bits(ophi)) and np.array_equal(_bits(out[0][1]), _bits(opi))
+
+
+def test_two_step_unavailable_is_reported():
+    m = O.model(33)  # odd N: no vector width fits
+    with _lat(m) as L:
+        assert not L.two_step()
+        with pytest.raises(pn.PhiNumError):
+            L.two_step(True)
+
+
+def test_gpu_matches_reference_fixtures():
+    # the HIP path against the reference's own outputs (tests/golden), no oracle
+    import glob
+    import os
+    for path in sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))):
+        z = np.load(path)
+        N = int(z["N"])
+        steps = int(z["steps"])
+        dtk, dtd, _ = pn.time_factors(float(z["T0"]), float(z["dtbase"]), float(z["a"]), steps)
+        with pn.Lattice(N, float(z["a"]), int(z["mass_mode"]), float(z["m2"]), int(z["lambda_mode"]),
+                        float(z["kappa"])) as L:
+            L.set_fields(z["phi_in"], z["pi_in"])
+            L.evolve(dtk, dtd)
+            gphi, gpi = L.get_fields()
+            assert np.array_equal(_bits(gphi), _bits(z["phi_out"])), path
+            assert np.array_equal(_bits(gpi), _bits(z["pi_out"])), path
+            ext = L.extrema()
+            assert (ext.min, ext.max, ext.maxabs) == tuple(z["ext"])
+            assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in z["extpos"])
+            np.testing.assert_allclose(L.volume_averages().as_array(), z["avg"], rtol=1e-13, atol=1e-15)
+            if int(z["islands"]):
+                rows, lab, _ = L.islands(0, float(z["thr"]), labels=True)
+                assert np.array_equal(lab, z["labels"])
+                cat = z["catalog"]
+                assert len(rows) == cat.shape[0]
+                for r, c in zip(rows, cat):
+                    assert (r.index, r.nn, r.peak_pos) == (int(c[0]), int(c[1]), int(c[3]))
+                    assert r.volume == c[2] and r.peak_value == c[4]
+                    np.testing.assert_allclose([r.avg_phi, r.pot_en, r.kin_en, r.grad_en], c[5:9],
+                                               rtol=1e-12, atol=1e-14)
+
+
 def test_non_pow2_spacing_and_other_couplings_bitwise():
     m = dict(N=20, a=0.37, mass_mode=-1, m2=0.7, lambda_mode=1, kappa=1.9, Lambda=3.0)
     phi, pi = _fields(20, 9, phi0=0.3)
