@@ -36,8 +36,8 @@ struct BatchDesc {
   int N, XT, YT, ZT, TY, ZB;
   int tile_start;  // first tile of this realization in the tile list
   int pow2, decayed;
-  double a2, inv_a2, c2, lmode, kappa;
-  float f_inv_a2, f_c2, f_c4, f_c6;
+  CoefF64 k64;
+  CoefF32 k32;
 };
 
 struct pn_batch {
@@ -148,12 +148,10 @@ __global__ __launch_bounds__(256) void k_batch_step(const T *__restrict__ phi_al
           const T left = (jj == 0) ? vxm : fc[(jj + W - 1) % W];
           T pj = p[jj];
           if constexpr (sizeof(T) == 8) {
-            CoefF64 k{d.a2, d.inv_a2, d.c2, d.lmode, d.kappa};
-            if (d.pow2) site_update_f64<true>(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
-            else site_update_f64<false>(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+            if (d.pow2) site_update_f64<true, DIV_PER_SITE>(d.k64, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+            else site_update_f64<false, DIV_PER_SITE>(d.k64, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
           } else {
-            CoefF32 k{d.f_inv_a2, d.f_c2, d.f_c4, d.f_c6};
-            site_update_f32(k, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
+            site_update_f32(d.k32, right, left, vyp[jj], vym[jj], fp[jj], fm[jj], fc[jj], pj, dk);
           }
           p[jj] = pj;
         }
@@ -204,16 +202,17 @@ static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
     d.XT = (q.N + 64 * W - 1) / (64 * W);
     d.YT = (q.N + d.TY - 1) / d.TY;
     d.ZT = (q.N + d.ZB - 1) / d.ZB;
-    d.a2 = q.a * q.a;
-    d.inv_a2 = 1.0 / d.a2;
-    d.pow2 = pow2d(d.a2);
-    d.c2 = q.mass_mode * q.m2;
-    d.lmode = (double)q.lambda_mode;
-    d.kappa = q.kappa;
-    d.f_inv_a2 = (float)(1.0 / d.a2);
-    d.f_c2 = (float)d.c2;
-    d.f_c4 = (float)(d.lmode / 6.0);
-    d.f_c6 = (float)(1.0 / (120.0 * q.kappa * q.kappa));
+    d.k64.a2 = q.a * q.a;
+    d.k64.inv_a2 = 1.0 / d.k64.a2;
+    d.pow2 = pow2d(d.k64.a2);
+    d.k64.c2 = q.mass_mode * q.m2;
+    d.k64.lmode = (double)q.lambda_mode;
+    d.k64.kappa = q.kappa;
+    coef64_recips(d.k64);
+    d.k32.inv_a2 = (float)(1.0 / d.k64.a2);
+    d.k32.c2 = (float)d.k64.c2;
+    d.k32.c4 = (float)(d.k64.lmode / 6.0);
+    d.k32.c6 = (float)(1.0 / (120.0 * q.kappa * q.kappa));
     d.tile_start = tiles;
     b->htile[r] = tiles;
     if (!mask || (*mask)[r]) tiles += d.XT * d.YT * d.ZT;

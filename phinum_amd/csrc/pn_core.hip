@@ -9,9 +9,11 @@
 // the reference implements with halo planes and Communication::exchange
 // (comm.cpp:63-78) and with mod() (util.cpp:12-18) is index arithmetic here.
 //
-// FP64 arithmetic keeps the reference's expression order and its IEEE
-// divisions (dynamics.cpp:80,84) and is compiled with -ffp-contract=off, so
-// each site's update is bit-identical to a strict build of the reference.
+// FP64 arithmetic keeps the reference's expression order and its correctly
+// rounded divisions (dynamics.cpp:80,84) -- by a run-constant divisor they are
+// computed in four FMA-pipe operations (pn_fastdiv.h) where the operands allow,
+// with the same result bits -- and is compiled with -ffp-contract=off, so each
+// site's update is bit-identical to a strict build of the reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *
     const T vym = phi[base + rowm + x];
     T p = __builtin_nontemporal_load(&pi[base + row + x]);
     if constexpr (sizeof(T) == 8) {
-      site_update_f64<POW2>(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
+      site_update_f64<POW2, DIV_PER_SITE>(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
     } else {
       site_update_f32(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
     }
@@ -158,17 +160,30 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
     const V vyp = *(const V *)&phi[base + rowp + x];
     const V vym = *(const V *)&phi[base + rowm + x];
     V p = __builtin_nontemporal_load((const V *)&pi[base + row + x]);
+    if constexpr (sizeof(T) == 8) {
+      bool ok = true;
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-      const T right = (j == W - 1) ? vxp : fc[(j + 1) % W];
-      const T left = (j == 0) ? vxm : fc[(j + W - 1) % W];
-      T pj = p[j];
-      if constexpr (sizeof(T) == 8) {
-        site_update_f64<POW2>(k, right, left, vyp[j], vym[j], fp[j], fm[j], fc[j], pj, dtk);
-      } else {
+      for (int j = 0; j < W; j++) ok = ok && fast_div_site(fc[j]);
+      with_div_mode(k, ok, [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? vxp : fc[(j + 1) % W];
+          const T left = (j == 0) ? vxm : fc[(j + W - 1) % W];
+          T pj = p[j];
+          site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], fp[j], fm[j],
+                                                      fc[j], pj, dtk);
+          p[j] = pj;
+        }
+      });
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; j++) {
+        const T right = (j == W - 1) ? vxp : fc[(j + 1) % W];
+        const T left = (j == 0) ? vxm : fc[(j + W - 1) % W];
+        T pj = p[j];
         site_update_f32(k, right, left, vyp[j], vym[j], fp[j], fm[j], fc[j], pj, dtk);
+        p[j] = pj;
       }
-      p[j] = pj;
     }
     __builtin_nontemporal_store(p, (V *)&pi[base + row + x]);
     if constexpr (MODE == 0) {
@@ -225,43 +240,69 @@ __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const
     const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
     V p = *(const V *)&pi[base + row + x];
     V f1;
+    auto stage_a = [&](auto div) {
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-      const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
-      const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
-      T pj = p[j];
-      if constexpr (sizeof(T) == 8)
-        site_update_f64<POW2>(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
-      else
-        site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
-      p[j] = pj;
-      f1[j] = an_c[j] + pj * dd0;
+      for (int j = 0; j < W; j++) {
+        const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
+        const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
+        T pj = p[j];
+        if constexpr (sizeof(T) == 8)
+          site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], an_p[j],
+                                                      an_m[j], an_c[j], pj, dk0);
+        else
+          site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
+        p[j] = pj;
+        f1[j] = an_c[j] + pj * dd0;
+      }
+    };
+    if constexpr (sizeof(T) == 8) {
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < W; j++) ok = ok && fast_div_site(an_c[j]);
+      with_div_mode(k, ok, stage_a);
+    } else {
+      stage_a(std::integral_constant<int, DIV_IEEE>());
     }
     T *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
     *(V *)&buf[ty * N + x] = f1;
-    __syncthreads();
     if (za >= z0 + 1 && inner) {
       const int zb = za - 1;
       const T *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
       const T bxp = bb[ty * N + xp], bxm = bb[ty * N + xm];
       const V byp = *(const V *)&bb[(ty + 1) * N + x], bym = *(const V *)&bb[(ty - 1) * N + x];
       V q = pih, f2;
+      auto stage_b = [&](auto div) {
 #pragma unroll
-      for (int j = 0; j < W; j++) {
-        const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
-        const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
-        T qj = q[j];
-        if constexpr (sizeof(T) == 8)
-          site_update_f64<POW2>(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
-        else
-          site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
-        q[j] = qj;
-        f2[j] = b_c[j] + qj * dd1;
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
+          const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
+          T qj = q[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, byp[j], bym[j], f1[j],
+                                                        b_m[j], b_c[j], qj, dk1);
+          else
+            site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
+          q[j] = qj;
+          f2[j] = b_c[j] + qj * dd1;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(b_c[j]);
+        with_div_mode(k, ok, stage_b);
+      } else {
+        stage_b(std::integral_constant<int, DIV_IEEE>());
       }
       const long long bbase = (long long)zb * P;
       __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
       __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
     }
+    // The only barrier of the iteration.  LDS holds two planes: stage A of
+    // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
+    // written one iteration earlier.  This barrier orders that write before
+    // the next iteration's stage B reads it, and this iteration's stage B
+    // reads before the next stage A overwrites the buffer they use.
     __syncthreads();
     b_m = b_c;
     b_c = f1;
@@ -319,6 +360,7 @@ static CoefF64 coef64(const pn_ctx *c) {
   k.c2 = c->k.c2;
   k.lmode = c->k.lmode;
   k.kappa = c->k.kappa;
+  coef64_recips(k);
   return k;
 }
 
