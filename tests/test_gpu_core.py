@@ -117,6 +117,36 @@ def test_non_pow2_spacing_and_other_couplings_bitwise():
     assert np.array_equal(_bits(gphi), _bits(ophi)) and np.array_equal(_bits(gpi), _bits(opi))
 
 
+@pytest.mark.parametrize("N,huge", [(16, False), (16, True), (15, False), (15, True), (64, False),
+                                    (64, True)])
+def test_division_outside_fast_range_bitwise(N, huge):
+    # sites whose |phi| is 0, -0, below 2^-100 (tiny, subnormal) or above
+    # 2^100 make their wave take the IEEE divisions; the rest of the lattice
+    # keeps the four-FMA form: both must give the oracle's bits (a tiny time
+    # step keeps the huge sites finite over both steps)
+    m = O.model(N)
+    phi, pi = _fields(N, 90 + N)
+    rng = np.random.default_rng(N)
+    idx = rng.choice(N ** 3, 12, replace=False)
+    phi[idx[:8]] = [0.0, -0.0, 1e-40, -3e-35, 5e-324, 1e-310, 2.0 ** -100, -2.0 ** -101]
+    dtbase = 0.1
+    if huge:
+        phi[idx[8:10]] = [1.5e30, -2.0 ** 100 * 1.5]
+        dtbase = 1e-160
+    ophi, opi = O.evolve(m, phi, pi, 2, dtbase=dtbase)
+    assert np.all(np.isfinite(ophi)) and np.all(np.isfinite(opi))
+    dtk, dtd, _ = pn.time_factors(0.0, dtbase, m["a"], 2)
+    for two in (True, False):
+        with _lat(m) as L:
+            if two and not L.two_step():
+                continue  # odd N: single steps only
+            L.two_step(two)
+            L.set_fields(phi, pi)
+            L.evolve(dtk, dtd)
+            gphi, gpi = L.get_fields()
+            assert np.array_equal(_bits(gphi), _bits(ophi)) and np.array_equal(_bits(gpi), _bits(opi))
+
+
 @pytest.mark.parametrize("N,steps,seed", [(16, 20, 61), (18, 10, 62), (64, 10, 63), (100, 4, 64)])
 def test_fused_step_bitwise_fp32(N, steps, seed):
     # the FP32 form against its binary32 oracle (vector path when N % 4 == 0)
