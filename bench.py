@@ -3,16 +3,22 @@
 Workload: the 256^3 FP64 drop-in configuration (dx = 0.5, dt = 0.05,
 V = phi^2/2 - phi^4/4 + 0.2 phi^6/6 in the reference's units, 10^4 steps with
 energy outputs every 200 steps).  `value` is FP64 lattice site-updates per
-second with the fields resident in HBM; the step kernel is timed with HIP
-events on the stream it runs on.  Beside it the line reports the FP32 form
-(FP32 fields, FP64 compensated energy sums) with its energy difference to the
-FP64 run, the 512^3 FP64 roofline case, and the CPU baseline.
+second with the fields resident in HBM, timed with HIP events on the stream
+the step kernels run on.  The steps run as the two-step (temporally blocked)
+kernel, bit-identical to single steps; `roofline` reports that kernel against
+its own algorithmic traffic (Phi and Pi read and written once per launch of
+two steps) and, beside it, the site-update rate against the one-step bound
+(32 B per site-update).  The line also carries the single-step kernel, the
+FP32 form (FP32 fields, FP64 compensated energy sums) with its energy
+difference to the FP64 run, the 512^3 case, the batched ensemble, the
+adjoint, and the CPU baseline.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-extra]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,14 +41,24 @@ def model(lam=1.0, g=0.2):
                 phi0=float(np.sqrt(L)) * 1.0, Lambda=L)
 
 
-def run_leg(pn, N, precision, steps, warmup, out_every, seed, m):
+def launches(steps, chunk, two_step):
+    """(two-step launches, single-step launches) for `steps` run in chunks."""
+    full, rest = divmod(steps, chunk)
+    sizes = [chunk] * full + ([rest] if rest else [])
+    if not two_step:
+        return 0, steps
+    return sum(n // 2 for n in sizes), sum(n % 2 for n in sizes)
+
+
+def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True):
     """Seeded initialize==3 IC on the device, then `steps` timed steps in chunks
     of `out_every` with an FP64 energy between chunks (outside the timing).
-    Returns (total event ms over `steps`, energies at each output)."""
+    Returns (total event ms over `steps`, energies at each output, two-step on)."""
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)
     energies = []
     with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
                     precision=precision) as L:
+        on = L.two_step() if two_step else L.two_step(False)  # on by default where a shape fits
         L.initial_conditions(pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
         if warmup:
             L.evolve(dtk[:warmup], dtd[:warmup])
@@ -54,7 +70,50 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m):
             total += L.evolve_timed(dtk[s:s + n], dtd[s:s + n])
             s += n
             energies.append(L.volume_averages().energy)
-    return total, np.array(energies)
+    return total, np.array(energies), on
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy) and written into profiles/<name>.md as
+    `traffic_bytes_per_launch = <int>`; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
+def roofline(kernel, N, ms_total, steps, chunk, two_step, esize, profile=None):
+    """Roofline of the dominant kernel: algorithmic bytes per launch (Phi and
+    Pi read once and written once: 4 * esize bytes per site) over the average
+    launch duration, plus the site-update rate against the one-step bound."""
+    n2, n1 = launches(steps, chunk, two_step)
+    sites = float(N) ** 3
+    algo = 4.0 * esize * sites
+    if n2:
+        # single-step launches in odd chunks take about half a two-step launch
+        per_launch_ms = ms_total / (n2 + 0.5 * n1)
+        steps_per_launch = 2
+    else:
+        per_launch_ms = ms_total / n1
+        steps_per_launch = 1
+    achieved = algo / (per_launch_ms * 1e-3) / 1e9
+    upd = sites * steps / (ms_total * 1e-3)
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+           "steps_per_launch": steps_per_launch, "algorithmic_bytes_per_launch": int(algo),
+           "avg_launch_ms": per_launch_ms,
+           "one_step_bound": {"bytes_per_site_update": 4 * esize,
+                              "site_updates_per_s_at_peak": HBM_PEAK_GBS * 1e9 / (4 * esize),
+                              "frac": upd * 4 * esize / 1e9 / HBM_PEAK_GBS}}
+    if profile:
+        t = profiled_traffic(profile)
+        if t:
+            out["traffic"] = t
+            out["traffic_over_algorithmic"] = t / algo
+            out["traffic_source"] = "profiles/%s.md" % profile
+    return out
 
 
 def batch_leg(pn, steps, out_every=500):
@@ -96,38 +155,56 @@ def batch_leg(pn, steps, out_every=500):
             "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0)))}
 
 
-def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, eps=1e-6):
-    """Reverse-mode gradient of the smooth oscillon fraction after `steps`
-    steps, in physical units (V = phi^2/2 - lam phi^4/4 + g phi^6/6): seeded
-    vacuum IC divided by sqrt(Lambda), checkpoints every K steps.  Checked
-    against central differences in lambda, g and one random phi0 direction."""
+def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, check_steps=1000, eps=1e-6):
+    """Reverse-mode gradient of the smooth oscillon fraction in physical units
+    (V = phi^2/2 - lam phi^4/4 + g phi^6/6): seeded vacuum IC divided by
+    sqrt(Lambda), checkpoints every K steps.  Timed over `steps` steps.  The
+    gradient is checked against central differences in lambda, g and one
+    random phi0 direction over `check_steps` steps, where the dynamics is not
+    yet chaotic enough to swamp a difference quotient; at `steps` steps the
+    lambda derivative is compared with differences at eps = 1e-6 ... 1e-8,
+    which converge to it as eps shrinks."""
     m = model(lam, g)
     with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"]) as L:
         L.initial_conditions(pn.ic_params(7, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
         phi, pi = L.get_fields()
     s = float(np.sqrt(m["Lambda"]))
     phi, pi = phi / s, pi / s
+    out = {"N": N, "steps": steps, "checkpoint_every": K}
+
+    def fwd(l_, g_, ph, dtk, dtd):
+        return pn.forward_observable(pn.adj_params(N, c2=1.0, c4=-l_, c6=g_), ph, pi, dtk, dtd)
+
+    # timing run over the full horizon
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
     p = pn.adj_params(N, c2=1.0, c4=-lam, c6=g, checkpoint_every=K)
     F, gphi, _, gc, res = pn.adjoint(p, phi, pi, dtk, dtd)
+    out.update({"F": F, "dF_dlambda": -gc[1], "dF_dg": gc[2], "forward_ms": res.forward_ms,
+                "backward_ms": res.backward_ms, "backward_over_forward": res.backward_ms / res.forward_ms,
+                "peak_device_bytes": res.peak_bytes})
+    conv = {}
+    for e in (1e-6, 1e-7, 1e-8):
+        conv["%g" % e] = (fwd(lam + e, g, phi, dtk, dtd) - fwd(lam - e, g, phi, dtk, dtd)) / (2 * e)
+    out["fd_dF_dlambda_vs_eps"] = conv
+    out["rel_err_fd_lambda_eps1e-8"] = abs(conv["1e-08"] - (-gc[1])) / abs(gc[1])
+    # gradient check over the shorter horizon
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], check_steps)
+    F, gphi, _, gc, _ = pn.adjoint(pn.adj_params(N, c2=1.0, c4=-lam, c6=g, checkpoint_every=K),
+                                   phi, pi, dtk, dtd)
     dF_dlam, dF_dg = -gc[1], gc[2]
-
-    def fwd(l_, g_, ph):
-        return pn.forward_observable(pn.adj_params(N, c2=1.0, c4=-l_, c6=g_), ph, pi, dtk, dtd)
-
-    fd_lam = (fwd(lam + eps, g, phi) - fwd(lam - eps, g, phi)) / (2 * eps)
-    fd_g = (fwd(lam, g + eps, phi) - fwd(lam, g - eps, phi)) / (2 * eps)
+    chk = {"steps": check_steps, "F": F, "dF_dlambda": dF_dlam, "dF_dg": dF_dg}
+    # central differences carry an eps^2 truncation term: shown at two eps
+    for e in (eps, eps / 10):
+        fd_lam = (fwd(lam + e, g, phi, dtk, dtd) - fwd(lam - e, g, phi, dtk, dtd)) / (2 * e)
+        fd_g = (fwd(lam, g + e, phi, dtk, dtd) - fwd(lam, g - e, phi, dtk, dtd)) / (2 * e)
+        chk["rel_err_fd_lambda_eps%g" % e] = abs(fd_lam - dF_dlam) / abs(dF_dlam)
+        chk["rel_err_fd_g_eps%g" % e] = abs(fd_g - dF_dg) / abs(dF_dg)
     v = np.random.default_rng(3).standard_normal(N ** 3)
-    fd_v = (fwd(lam, g, phi + eps * v) - fwd(lam, g, phi - eps * v)) / (2 * eps)
+    fd_v = (fwd(lam, g, phi + eps * v, dtk, dtd) - fwd(lam, g, phi - eps * v, dtk, dtd)) / (2 * eps)
     ad_v = float(np.dot(gphi, v))
-    return {"N": N, "steps": steps, "checkpoint_every": K, "F": F,
-            "dF_dlambda": dF_dlam, "dF_dg": dF_dg,
-            "rel_err_fd_lambda": abs(fd_lam - dF_dlam) / abs(dF_dlam),
-            "rel_err_fd_g": abs(fd_g - dF_dg) / abs(dF_dg),
-            "rel_err_fd_phi0_direction": abs(fd_v - ad_v) / abs(ad_v),
-            "forward_ms": res.forward_ms, "backward_ms": res.backward_ms,
-            "backward_over_forward": res.backward_ms / res.forward_ms,
-            "peak_device_bytes": res.peak_bytes}
+    chk["rel_err_fd_phi0_direction_eps%g" % eps] = abs(fd_v - ad_v) / abs(ad_v)
+    out["fd_check"] = chk
+    return out
 
 
 def cpu_baseline(N, m, target_s=12.0):
@@ -161,6 +238,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--out-every", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--single-step", action="store_true", help="time the one-step kernel only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -185,47 +263,56 @@ def main():
     import phinum_amd as pn
 
     N, K, W = args.size, args.steps, args.warmup
+    two = not args.single_step
     m = model()
-    log("fp64 %d^3: %d warmup + %d timed steps" % (N, W, K))
+    log("fp64 %d^3: %d warmup + %d timed steps (two-step kernel %s)" % (N, W, K, "on" if two else "off"))
     t0 = time.time()
-    ms64, e64 = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m)
-    log("fp64 done in %.1fs wall, %.3f ms/step" % (time.time() - t0, ms64 / K))
+    ms64, e64, on = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m, two_step=two)
+    log("fp64 done in %.1fs wall, %.4f ms/step" % (time.time() - t0, ms64 / K))
     sites = float(N) ** 3
     value = sites * K / (ms64 * 1e-3)
-    ms_step = ms64 / K
-    achieved = 32.0 * sites / (ms_step * 1e-3) / 1e9
+    kname = ("k_step2<double, CoefF64, POW2, W, RT> (two leapfrog steps per pass)" if on else
+             "k_step_vec<double, CoefF64, POW2, 2, 4, ZB, 0> (one fused step)")
+    prof = ("k_step2_fp64_%dcube" if on else "k_step_vec_fp64_%dcube") % N
     line = {
         "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "warmup": W, "ms_per_step": ms64 / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded initialize==3 vacuum field (seed 1234, phi0=1 in physical units, "
                 "modes below k_max/2), generated on the device",
         "config": {"workload": "fp64 leapfrog %d^3, dx=0.5, dt=0.05, lambda=1, g=0.2, %d steps, "
                                "energy every %d steps" % (N, K, args.out_every),
-                   "N": N, "out_every": args.out_every},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step<double> (fused kick+drift)", "bytes_per_site": 32},
+                   "N": N, "out_every": args.out_every, "two_step_kernel": bool(on)},
+        "roofline": roofline(kname, N, ms64, K, args.out_every, on, 8, prof),
         "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
     }
     if not args.no_extra:
+        log("fp64 %d^3 single-step kernel" % N)
+        K1 = min(K, 2000)
+        ms1, _, _ = run_leg(pn, N, pn.PN_FP64, K1, 10, args.out_every, 1234, m, two_step=False)
+        line["single_step_kernel"] = {
+            "value": sites * K1 / (ms1 * 1e-3), "unit": "site-updates/s", "steps": K1,
+            "ms_per_step": ms1 / K1,
+            "roofline": roofline("k_step_vec<double, CoefF64, true, 2, 4, 32, 0>", N, ms1, K1,
+                                 args.out_every, False, 8, "k_step_vec_fp64_%dcube" % N)}
         log("fp32 %d^3" % N)
-        ms32, e32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
+        ms32, e32, on32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
         v32 = sites * K / (ms32 * 1e-3)
         line["fp32_comp"] = {
             "value": v32, "unit": "site-updates/s", "ms_per_step": ms32 / K,
             "ratio_vs_fp64": v32 / value,
             "dE_over_E_vs_fp64_max": float(np.max(np.abs(e32 - e64) / np.abs(e64))),
             "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
-            "roofline_frac": (16.0 * sites / (ms32 / K * 1e-3) / 1e9) / HBM_PEAK_GBS,
+            "roofline": roofline("k_step2<float, CoefF32, true, 4, 8>", N, ms32, K, args.out_every,
+                                 on32, 4),
         }
         N5 = 512
-        log("fp64 %d^3 roofline case" % N5)
-        ms5, _ = run_leg(pn, N5, pn.PN_FP64, 200, 20, 200, 5, m)
-        a5 = 32.0 * float(N5) ** 3 / (ms5 / 200 * 1e-3) / 1e9
-        line["roofline_512"] = {"bound": "hbm", "achieved": a5, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": a5 / HBM_PEAK_GBS, "ms_per_step": ms5 / 200,
-                                "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3)}
+        log("fp64 %d^3" % N5)
+        ms5, _, on5 = run_leg(pn, N5, pn.PN_FP64, 200, 20, 200, 5, m)
+        line["fp64_512"] = {"ms_per_step": ms5 / 200,
+                            "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
+                            "roofline": roofline("k_step2<double, CoefF64, true, 4, 6>", N5, ms5, 200,
+                                                 200, on5, 8, "k_step2_fp64_512cube")}
         log("batch of 64 realizations")
         line["batch_64"] = batch_leg(pn, args.batch_steps)
         log("adjoint 128^3")
@@ -235,6 +322,7 @@ def main():
         cb = cpu_baseline(N, m)
         if cb:
             line["cpu_baseline"] = cb
+            line["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(line), flush=True)
 
 
