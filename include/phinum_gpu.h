@@ -245,12 +245,20 @@ int pn_batch_sync(pn_batch *b);
 /* Freezes the flagged realizations (the false_vacuum_decayed break of
  * main.cpp:85-89): their fields stop changing. flags: R ints. */
 int pn_batch_set_decayed(pn_batch *b, const int *flags);
-/* Tile shape for every realization of size N: rows per tile, planes per z block. */
+/* Tile shape of the single-step kernel for every realization of size N: rows
+ * per tile, planes per z block. */
 int pn_batch_set_tiles(pn_batch *b, int N, int rows, int zblock);
 /* Times candidate tile shapes per size class on the live batch (state is
  * restored afterwards) and keeps the fastest.  out receives nclasses triples
- * (N, rows, zblock); pass NULL to skip. */
+ * (N, rows, zblock); pass NULL to skip.  In two-step mode only the planes per
+ * tile are tuned and rows is the two-step tile's thread rows. */
 int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nclasses);
+/* PN_OPT_TWO_STEP for a batch (no reference counterpart): 1 (default when
+ * every realization fits: N a multiple of the vector width, N/width <= 768)
+ * advances pairs of steps in one launch; 0 runs single steps.  Both give
+ * bit-identical fields. */
+int pn_batch_set_option(pn_batch *b, int option, int value);
+int pn_batch_get_option(pn_batch *b, int option, int *value);
 
 /* ---- differentiable leapfrog (no reference counterpart) ----------------- */
 

@@ -132,6 +132,8 @@ def lib():
         "pn_batch_set_decayed": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "pn_batch_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "pn_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]),
+        "pn_batch_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "pn_batch_get_option": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int)]),
         "pn_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "pn_get_option": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int)]),
         "pn_forward_observable": (C.c_int, [P(PnAdjParams), dp, dp, C.c_int, dp, dp, dp]),
@@ -437,6 +439,14 @@ class Batch:
 
     def set_tiles(self, N, rows, zblock):
         self._chk(self.L.pn_batch_set_tiles(self.h, int(N), int(rows), int(zblock)))
+
+    def two_step(self, on=None):
+        """Gets or sets PN_OPT_TWO_STEP (pairs of steps per launch) for the batch."""
+        if on is not None:
+            self._chk(self.L.pn_batch_set_option(self.h, PN_OPT_TWO_STEP, int(bool(on))))
+        v = C.c_int()
+        self._chk(self.L.pn_batch_get_option(self.h, PN_OPT_TWO_STEP, C.byref(v)))
+        return bool(v.value)
 
     def autotune(self, trial_steps=10):
         out = np.zeros(3 * 64, dtype=np.int32)

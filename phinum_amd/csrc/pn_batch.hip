@@ -13,6 +13,11 @@
 // device function as the single-lattice kernel (bit-identical results).
 // Sequences of steps are captured once into a HIP graph and replayed.
 //
+// Two-step mode (the default when every realization fits): one launch per
+// pair of steps, each tile running the single-lattice two-step tile code
+// (step2_tile) on full x rows; a block of kB2 threads covers kB2*W/N rows of
+// an N^3 realization.  Pi then alternates between two buffers.
+//
 // Tile shape (rows per tile, planes per z block) is chosen per size class by
 // an autotuner that times the candidates on the live batch and restores it.
 //
@@ -36,6 +41,8 @@ struct BatchDesc {
   int N, XT, YT, ZT, TY, ZB;
   int tile_start;  // first tile of this realization in the tile list
   int pow2, decayed;
+  int RT2, YT2, ZB2, ZT2;  // two-step tiles: RT2 thread rows (RT2-2 output rows), ZB2 planes
+  int tile_start2;         // first tile in the two-step tile list
   CoefF64 k64;
   CoefF32 k32;
 };
@@ -47,8 +54,15 @@ struct pn_batch {
   std::vector<long long> off;
   long long total = 0;
   void *phi[2] = {nullptr, nullptr};
-  void *pi = nullptr;
+  void *pib[2] = {nullptr, nullptr};  // Pi; the second buffer only in two-step mode
+  int pcur = 0;                        // pib[pcur] holds the current Pi
   int cur = 0;
+  int step2_ok = 0;                    // every realization fits a two-step tile
+  int step2 = 0;                       // two-step mode on
+  std::map<int, int> zb2;              // N -> planes per two-step tile
+  std::vector<int> htile2;
+  int *dtile2 = nullptr;
+  int total_tiles2 = 0;
   hipStream_t stream = nullptr;
   std::vector<BatchDesc> hdesc;
   BatchDesc *ddesc = nullptr;
@@ -60,7 +74,7 @@ struct pn_batch {
   int *dbase = nullptr;  // step base for graph replays
   std::map<int, std::pair<int, int>> tiles;  // N -> (TY, ZB)
   std::vector<pn_ctx *> views;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [cur][pcur]
   int graph_len = 0;
   std::string err;
 };
@@ -167,6 +181,73 @@ __global__ __launch_bounds__(256) void k_batch_step(const T *__restrict__ phi_al
   }
 }
 
+// Two steps per launch for all realizations: a grid-stride walk over the
+// two-step tile list.  Block = kB2 threads, thread t -> (t % (N/W), t / (N/W))
+// of the tile's realization; threads beyond its RT2 rows only join barriers.
+static constexpr int kB2 = 768;
+
+template <typename T, int W>
+__global__ __launch_bounds__(kB2) void k_batch_step2(const T *__restrict__ phi_all,
+                                                     const T *__restrict__ pi_all,
+                                                     T *__restrict__ pio_all,
+                                                     T *__restrict__ phin_all,
+                                                     const BatchDesc *__restrict__ desc,
+                                                     const int *__restrict__ tile_start, int R,
+                                                     int total_tiles,
+                                                     const double *__restrict__ dtk_tab,
+                                                     const double *__restrict__ dtd_tab, int cap,
+                                                     const int *__restrict__ base_ptr, int j) {
+  typedef typename VecT<T, W>::type V;
+  extern __shared__ unsigned char lds_raw[];
+  const int step = *base_ptr + j;  // this launch advances steps `step` and `step + 1`
+  for (int b = blockIdx.x; b < total_tiles; b += gridDim.x) {
+    int lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tile_start[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    const BatchDesc d = desc[lo];
+    const int t = b - d.tile_start2;
+    const int yt = t % d.YT2, zt = t / d.YT2;
+    const int N = d.N, RT = d.RT2, tpr = N / W;
+    const int tx = (int)threadIdx.x % tpr, ty = (int)threadIdx.x / tpr;
+    const bool active = ty < RT;
+    const int y0 = yt * (RT - 2), z0 = zt * d.ZB2, z1 = min(z0 + d.ZB2, N);
+    const T *phi = phi_all + d.off;
+    const T *pi = pi_all + d.off;
+    T *pio = pio_all + d.off;
+    T *phin = phin_all + d.off;
+    if (d.decayed) {
+      // frozen (main.cpp:85-89 ends a decayed run): the tile's output rows
+      // keep their Phi and Pi in the buffers the next launch reads
+      const int y = y0 + ty - 1;
+      if (active && ty >= 1 && ty <= RT - 2 && y < N) {
+        const long long P = (long long)N * N;
+        for (int z = z0; z < z1; z++) {
+          const long long i = z * P + (long long)y * N + W * tx;
+          *(V *)&phin[i] = *(const V *)&phi[i];
+          *(V *)&pio[i] = *(const V *)&pi[i];
+        }
+      }
+      continue;
+    }
+    const long long tb = (long long)lo * cap + step;
+    const T dk0 = (T)dtk_tab[tb], dd0 = (T)dtd_tab[tb], dk1 = (T)dtk_tab[tb + 1], dd1 = (T)dtd_tab[tb + 1];
+    if constexpr (sizeof(T) == 8) {
+      if (d.pow2)
+        step2_tile<T, CoefF64, true, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+                                        z0, z1, d.k64, dk0, dd0, dk1, dd1);
+      else
+        step2_tile<T, CoefF64, false, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active,
+                                         y0, z0, z1, d.k64, dk0, dd0, dk1, dd1);
+    } else {
+      step2_tile<T, CoefF32, true, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+                                      z0, z1, d.k32, dk0, dd0, dk1, dd1);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -180,13 +261,24 @@ static int vec_width(const pn_batch *b) { return b->precision == PN_FP64 ? 2 : 4
 
 static std::pair<int, int> default_tiles(int N) { return {4, N >= 256 ? 16 : 32}; }
 
+// two-step rows per tile for an N^3 realization (0 when it does not fit)
+static int step2_rows(const pn_batch *b, int N) {
+  const int W = vec_width(b);
+  if (N % W != 0 || N / W > kB2) return 0;
+  const int rt = std::min(kB2 / (N / W), N + 2);
+  return rt >= 3 ? rt : 0;
+}
+
+static size_t step2_lds(const pn_batch *b) { return (size_t)2 * kB2 * vec_width(b) * b->esize; }
+
 // rebuild descriptors and the tile list; mask (optional) selects the
 // realizations that get tiles
 static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
   const int W = vec_width(b);
   b->hdesc.resize(b->R);
   b->htile.assign(b->R + 1, 0);
-  int tiles = 0;
+  b->htile2.assign(b->R + 1, 0);
+  int tiles = 0, tiles2 = 0;
   for (int r = 0; r < b->R; r++) {
     const pn_realization &q = b->re[r];
     BatchDesc &d = b->hdesc[r];
@@ -216,25 +308,43 @@ static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
     d.tile_start = tiles;
     b->htile[r] = tiles;
     if (!mask || (*mask)[r]) tiles += d.XT * d.YT * d.ZT;
+    d.RT2 = step2_rows(b, q.N);
+    if (d.RT2) {
+      auto z2 = b->zb2.find(q.N);
+      d.ZB2 = std::min(z2 != b->zb2.end() ? z2->second : 16, q.N);
+      d.YT2 = (q.N + d.RT2 - 3) / (d.RT2 - 2);
+      d.ZT2 = (q.N + d.ZB2 - 1) / d.ZB2;
+    }
+    d.tile_start2 = tiles2;
+    b->htile2[r] = tiles2;
+    if (d.RT2 && (!mask || (*mask)[r])) tiles2 += d.YT2 * d.ZT2;
   }
   b->htile[b->R] = tiles;
   b->total_tiles = tiles;
+  b->htile2[b->R] = tiles2;
+  b->total_tiles2 = tiles2;
   PB_HIP(b, hipMemcpyAsync(b->ddesc, b->hdesc.data(), sizeof(BatchDesc) * b->R, hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(b->dtile, b->htile.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(b->dtile2, b->htile2.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipStreamSynchronize(b->stream));
   return PN_OK;
 }
 
 static void drop_graphs(pn_batch *b) {
   for (int i = 0; i < 2; i++)
-    if (b->gexec[i]) {
-      (void)hipGraphExecDestroy(b->gexec[i]);
-      b->gexec[i] = nullptr;
-    }
+    for (int k = 0; k < 2; k++)
+      if (b->gexec[i][k]) {
+        (void)hipGraphExecDestroy(b->gexec[i][k]);
+        b->gexec[i][k] = nullptr;
+      }
 }
 
+// views follow the current Phi parity and the current Pi buffer
 static void sync_views(pn_batch *b) {
-  for (pn_ctx *v : b->views) v->cur = b->cur;
+  for (int r = 0; r < b->R; r++) {
+    b->views[r]->cur = b->cur;
+    b->views[r]->pi = (char *)b->pib[b->pcur] + b->off[r] * b->esize;
+  }
 }
 
 extern "C" int pn_batch_destroy(pn_batch *b) {
@@ -244,9 +354,10 @@ extern "C" int pn_batch_destroy(pn_batch *b) {
   drop_graphs(b);
   for (pn_ctx *v : b->views) pn_destroy(v);
   for (int i = 0; i < 2; i++) (void)hipFree(b->phi[i]);
-  (void)hipFree(b->pi);
+  for (int i = 0; i < 2; i++) (void)hipFree(b->pib[i]);
   (void)hipFree(b->ddesc);
   (void)hipFree(b->dtile);
+  (void)hipFree(b->dtile2);
   (void)hipFree(b->dtk);
   (void)hipFree(b->dtd);
   (void)hipFree(b->dbase);
@@ -297,13 +408,23 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
   const size_t bytes = (size_t)total * b->esize;
   for (int i = 0; i < 2; i++)
     if (hipMalloc(&b->phi[i], bytes) != hipSuccess) return fail(PN_ERR_NOMEM, "pn_batch_create: field allocation failed");
-  if (hipMalloc(&b->pi, bytes) != hipSuccess) return fail(PN_ERR_NOMEM, "pn_batch_create: field allocation failed");
+  if (hipMalloc(&b->pib[0], bytes) != hipSuccess) return fail(PN_ERR_NOMEM, "pn_batch_create: field allocation failed");
+  b->step2_ok = 1;
+  for (int r = 0; r < R; r++)
+    if (!step2_rows(b, b->re[r].N)) b->step2_ok = 0;
+  // the second Pi buffer of two-step mode; without room the batch runs single steps
+  if (b->step2_ok && hipMalloc(&b->pib[1], bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b->pib[1] = nullptr;
+    b->step2_ok = 0;
+  }
+  b->step2 = b->step2_ok;
   if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * R) != hipSuccess || hipMalloc(&b->dtile, sizeof(int) * (R + 1)) != hipSuccess ||
-      hipMalloc(&b->dbase, sizeof(int)) != hipSuccess)
+      hipMalloc(&b->dtile2, sizeof(int) * (R + 1)) != hipSuccess || hipMalloc(&b->dbase, sizeof(int)) != hipSuccess)
     return fail(PN_ERR_NOMEM, "pn_batch_create: descriptor allocation failed");
   for (int i = 0; i < 2; i++)
     if (hipMemsetAsync(b->phi[i], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
-  if (hipMemsetAsync(b->pi, 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
+  if (hipMemsetAsync(b->pib[0], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
   b->hdesc.assign(R, BatchDesc());
   if (upload_desc(b) != PN_OK) return fail(PN_ERR_HIP, "pn_batch_create: descriptor upload failed");
   for (int r = 0; r < R; r++) {
@@ -311,7 +432,7 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
     pn_params p{q.N, q.a, q.mass_mode, q.m2, q.lambda_mode, q.kappa, precision, device};
     char *p0 = (char *)b->phi[0] + b->off[r] * b->esize;
     char *p1 = (char *)b->phi[1] + b->off[r] * b->esize;
-    char *pp = (char *)b->pi + b->off[r] * b->esize;
+    char *pp = (char *)b->pib[0] + b->off[r] * b->esize;
     pn_ctx *v = nullptr;
     if (pn_make_view(&p, p0, p1, pp, 0, b->stream, &v) != PN_OK)
       return fail(PN_ERR_NOMEM, "pn_batch_create: view creation failed");
@@ -323,7 +444,7 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
 
 extern "C" int pn_batch_view(pn_batch *b, int r, pn_ctx **view) {
   if (!b || !view || r < 0 || r >= b->R) return bfail(b, PN_ERR_ARG, "pn_batch_view: bad argument");
-  b->views[r]->cur = b->cur;
+  sync_views(b);
   *view = b->views[r];
   return PN_OK;
 }
@@ -349,32 +470,55 @@ static int grid_blocks(const pn_batch *b) {
   return std::max(1, std::min(b->total_tiles, 256 * 8));
 }
 
-static void launch_batch_step(pn_batch *b, int parity, int j) {
-  const int W = vec_width(b);
+static void launch_batch_step(pn_batch *b, int parity, int pparity, int j) {
   const int grid = grid_blocks(b);
   if (b->precision == PN_FP64)
     hipLaunchKernelGGL((k_batch_step<double, 2>), dim3(grid), dim3(64, 4), 0, b->stream,
-                       (const double *)b->phi[parity], (double *)b->pi, (double *)b->phi[parity ^ 1],
-                       b->ddesc, b->dtile, b->R, b->total_tiles, b->dtk, b->dtd, b->table_cap,
-                       b->dbase, j);
+                       (const double *)b->phi[parity], (double *)b->pib[pparity],
+                       (double *)b->phi[parity ^ 1], b->ddesc, b->dtile, b->R, b->total_tiles,
+                       b->dtk, b->dtd, b->table_cap, b->dbase, j);
   else
     hipLaunchKernelGGL((k_batch_step<float, 4>), dim3(grid), dim3(64, 4), 0, b->stream,
-                       (const float *)b->phi[parity], (float *)b->pi, (float *)b->phi[parity ^ 1],
-                       b->ddesc, b->dtile, b->R, b->total_tiles, b->dtk, b->dtd, b->table_cap,
-                       b->dbase, j);
-  (void)W;
+                       (const float *)b->phi[parity], (float *)b->pib[pparity],
+                       (float *)b->phi[parity ^ 1], b->ddesc, b->dtile, b->R, b->total_tiles,
+                       b->dtk, b->dtd, b->table_cap, b->dbase, j);
 }
 
-static constexpr int kGraphLen = 100;  // steps per captured graph (even: parity preserved)
+// steps j and j+1 of the tables: Phi parity -> parity^1, Pi pib[pparity] -> pib[pparity^1]
+static void launch_batch_step2(pn_batch *b, int parity, int pparity, int j) {
+  // two blocks of kB2 threads per CU
+  const int grid = std::max(1, std::min(b->total_tiles2, 256 * 2));
+  const size_t lds = step2_lds(b);
+  if (b->precision == PN_FP64)
+    hipLaunchKernelGGL((k_batch_step2<double, 2>), dim3(grid), dim3(kB2), lds, b->stream,
+                       (const double *)b->phi[parity], (const double *)b->pib[pparity],
+                       (double *)b->pib[pparity ^ 1], (double *)b->phi[parity ^ 1], b->ddesc,
+                       b->dtile2, b->R, b->total_tiles2, b->dtk, b->dtd, b->table_cap, b->dbase, j);
+  else
+    hipLaunchKernelGGL((k_batch_step2<float, 4>), dim3(grid), dim3(kB2), lds, b->stream,
+                       (const float *)b->phi[parity], (const float *)b->pib[pparity],
+                       (float *)b->pib[pparity ^ 1], (float *)b->phi[parity ^ 1], b->ddesc,
+                       b->dtile2, b->R, b->total_tiles2, b->dtk, b->dtd, b->table_cap, b->dbase, j);
+}
 
-static int ensure_graph(pn_batch *b, int parity) {
-  if (b->gexec[parity] && b->graph_len == kGraphLen) return PN_OK;
+// steps per captured graph: 100 single steps or 50 two-step launches, both
+// even, so a replay leaves the Phi parity and the Pi buffer where it found them
+static constexpr int kGraphLen = 100;
+
+static int ensure_graph(pn_batch *b, int parity, int pparity) {
+  if (b->gexec[parity][pparity] && b->graph_len == kGraphLen) return PN_OK;
   hipGraph_t g = nullptr;
   PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int j = 0; j < kGraphLen; j++) launch_batch_step(b, parity ^ (j & 1), j);
+  if (b->step2)
+    for (int j = 0; j < kGraphLen; j += 2) {
+      const int flip = (j / 2) & 1;
+      launch_batch_step2(b, parity ^ flip, pparity ^ flip, j);
+    }
+  else
+    for (int j = 0; j < kGraphLen; j++) launch_batch_step(b, parity ^ (j & 1), pparity, j);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_evolve: graph capture: ", hipGetErrorString(e));
-  e = hipGraphInstantiate(&b->gexec[parity], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&b->gexec[parity][pparity], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_evolve: graph instantiate: ", hipGetErrorString(e));
   b->graph_len = kGraphLen;
@@ -411,14 +555,21 @@ static int run_steps(pn_batch *b, int nsteps, int use_graph) {
   while (s < nsteps) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->dbase, s);
     if (use_graph && nsteps - s >= kGraphLen) {
-      int rc = ensure_graph(b, b->cur);
+      int rc = ensure_graph(b, b->cur, b->pcur);
       if (rc != PN_OK) return rc;
-      PB_HIP(b, hipGraphLaunch(b->gexec[b->cur], b->stream));
-      s += kGraphLen;  // even length: parity unchanged
+      PB_HIP(b, hipGraphLaunch(b->gexec[b->cur][b->pcur], b->stream));
+      s += kGraphLen;  // even length: parity and Pi buffer unchanged
     } else {
       const int n = nsteps - s;
-      for (int j = 0; j < n; j++) {
-        launch_batch_step(b, b->cur, j);
+      int j = 0;
+      if (b->step2)
+        for (; j + 1 < n; j += 2) {
+          launch_batch_step2(b, b->cur, b->pcur, j);
+          b->cur ^= 1;
+          b->pcur ^= 1;
+        }
+      for (; j < n; j++) {
+        launch_batch_step(b, b->cur, b->pcur, j);
         b->cur ^= 1;
       }
       s += n;
@@ -455,7 +606,7 @@ extern "C" int pn_batch_evolve_timed(pn_batch *b, int nsteps, const double *dtk,
   // tables are uploaded (and graphs captured) before the timed region
   int rc = upload_tables(b, nsteps, dtk, dtd);
   if (rc != PN_OK) return rc;
-  if (use_graph && nsteps >= kGraphLen && (rc = ensure_graph(b, b->cur)) != PN_OK) return rc;
+  if (use_graph && nsteps >= kGraphLen && (rc = ensure_graph(b, b->cur, b->pcur)) != PN_OK) return rc;
   hipEvent_t e0, e1;
   PB_HIP(b, hipEventCreate(&e0));
   PB_HIP(b, hipEventCreate(&e1));
@@ -488,9 +639,10 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
     (void)hipFree(save_phi);
     return bfail(b, PN_ERR_NOMEM, "pn_batch_autotune: no room for the state backup");
   }
-  const int saved_cur = b->cur;
+  const int saved_cur = b->cur, saved_pcur = b->pcur;
   PB_HIP(b, hipMemcpyAsync(save_phi, b->phi[saved_cur], bytes, hipMemcpyDeviceToDevice, b->stream));
-  PB_HIP(b, hipMemcpyAsync(save_pi, b->pi, bytes, hipMemcpyDeviceToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(save_pi, b->pib[saved_pcur], bytes, hipMemcpyDeviceToDevice, b->stream));
+  if (b->step2) trial_steps += trial_steps & 1;  // whole two-step launches
   std::vector<double> tk((size_t)b->R * trial_steps, 1e-3), td((size_t)b->R * trial_steps, 1e-3);
   std::vector<int> classes;
   for (auto &q : b->re)
@@ -504,11 +656,16 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
     std::vector<int> mask(b->R, 0);
     for (int r = 0; r < b->R; r++) mask[r] = b->re[r].N == N;
     double best = 1e30;
-    std::pair<int, int> bestp = default_tiles(N);
-    for (int ty : rows_c)
+    std::pair<int, int> bestp = b->step2 ? std::make_pair(step2_rows(b, N), 16) : default_tiles(N);
+    // two-step mode tunes the planes per tile (its rows follow from N);
+    // single steps tune rows and planes
+    const int nrows = b->step2 ? 1 : 4;
+    for (int ri = 0; ri < nrows; ri++)
       for (int zb : zb_c) {
-        if (ty > N || zb > N) continue;
-        b->tiles[N] = {ty, zb};
+        const int ty = b->step2 ? step2_rows(b, N) : rows_c[ri];
+        if ((!b->step2 && ty > N) || zb > N) continue;
+        if (b->step2) b->zb2[N] = zb;
+        else b->tiles[N] = {ty, zb};
         if ((rc = upload_desc(b, &mask)) != PN_OK) break;
         double ms = 0;
         if ((rc = pn_batch_evolve(b, trial_steps, tk.data(), td.data(), 0)) != PN_OK) break;  // warm
@@ -519,7 +676,8 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
         }
       }
     if (rc != PN_OK) break;
-    b->tiles[N] = bestp;
+    if (b->step2) b->zb2[N] = bestp.second;
+    else b->tiles[N] = bestp;
     results.push_back(N);
     results.push_back(bestp.first);
     results.push_back(bestp.second);
@@ -527,8 +685,9 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
   // restore the state and the full tile list
   hipError_t e = hipStreamSynchronize(b->stream);
   b->cur = saved_cur;
+  b->pcur = saved_pcur;
   if (e == hipSuccess) e = hipMemcpyAsync(b->phi[saved_cur], save_phi, bytes, hipMemcpyDeviceToDevice, b->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(b->pi, save_pi, bytes, hipMemcpyDeviceToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->pib[saved_pcur], save_pi, bytes, hipMemcpyDeviceToDevice, b->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
   (void)hipFree(save_phi);
   (void)hipFree(save_pi);
@@ -539,5 +698,27 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
   if ((rc = upload_desc(b)) != PN_OK) return rc;
   if (nclasses) *nclasses = (int)classes.size();
   if (out) std::copy(results.begin(), results.end(), out);
+  return PN_OK;
+}
+
+extern "C" int pn_batch_set_option(pn_batch *b, int option, int value) {
+  if (!b) return bfail(b, PN_ERR_ARG, "pn_batch_set_option: NULL batch");
+  if (option != PN_OPT_TWO_STEP) return bfail(b, PN_ERR_ARG, "pn_batch_set_option: unknown option");
+  if (value && !b->step2_ok)
+    return bfail(b, PN_ERR_STATE, "pn_batch_set_option: a realization does not fit a two-step tile "
+                                  "(N % vector width, or N/width above the block size)");
+  if ((value != 0) != (b->step2 != 0)) {
+    PB_HIP(b, hipSetDevice(b->device));
+    PB_HIP(b, hipStreamSynchronize(b->stream));
+    b->step2 = value != 0;
+    drop_graphs(b);
+  }
+  return PN_OK;
+}
+
+extern "C" int pn_batch_get_option(pn_batch *b, int option, int *value) {
+  if (!b || !value) return bfail(b, PN_ERR_ARG, "pn_batch_get_option: NULL argument");
+  if (option != PN_OPT_TWO_STEP) return bfail(b, PN_ERR_ARG, "pn_batch_get_option: unknown option");
+  *value = b->step2;
   return PN_OK;
 }

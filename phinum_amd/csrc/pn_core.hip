@@ -197,121 +197,23 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
   }
 }
 
-// Two leapfrog steps per pass over HBM (temporal blocking).  A block owns full
-// x rows (N/W threads of W sites each) and RT thread rows.  Stage A advances
-// step n on all RT rows, i.e. the block's RT-2 output rows plus one halo row on
-// each side; stage B advances step n+1 on the inner RT-2 rows, reading the
-// intermediate Phi of the current plane from LDS and the planes above and
-// below from registers.  The z loop runs stage A one plane ahead of stage B,
-// and one extra plane at each end of the block's z range.  Halo rows and planes
-// are recomputed by both neighbouring blocks with the same arithmetic, so the
-// result is bit-identical to two single steps.  Pi is read from `pi` and
-// written to `pio` (a second buffer): a neighbouring block's stage A still
-// needs the old Pi of the rows and planes this block writes.
+// Two leapfrog steps per pass over HBM (temporal blocking): one block per
+// tile of RT full x rows (N/W threads of W sites each) and ZB planes; the
+// tile's work is step2_tile.  Needs N % W == 0, (N/W)*RT <= 1024 and
+// 2*RT*N values of LDS.
 template <typename T, typename C, bool POW2, int W, int RT>
 __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const T *__restrict__ pi,
                                                 T *__restrict__ pio, T *__restrict__ phin, int N,
                                                 int ZB, C k, T dk0, T dd0, T dk1, T dd1) {
-  typedef typename VecT<T, W>::type V;
   extern __shared__ unsigned char lds_raw[];
-  T *lds = (T *)lds_raw;  // two planes x RT rows x N
-  constexpr int TY = RT - 2;
-  const int YT = (N + TY - 1) / TY;
+  const int YT = (N + RT - 3) / (RT - 2);
   // y-neighbouring tiles (which re-read each other's edge rows) on one XCD
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int yt = tile % YT, zt = tile / YT;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int x = W * tx;
-  const int y0 = yt * TY;
-  int y = y0 - 1 + ty;
-  y = y < 0 ? y + N : (y >= N ? y - N : y);
-  const bool inner = ty >= 1 && ty <= TY && (y0 + ty - 1) < N;
-  const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
-  const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
-  const long long P = (long long)N * N;
-  const int row = y * N, rowp = yp * N, rowm = ym * N;
-  const int z0 = zt * ZB, z1 = min(z0 + ZB, N);
-  auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
-  V an_m = *(const V *)&phi[wz(z0 - 2) * P + row + x];
-  V an_c = *(const V *)&phi[wz(z0 - 1) * P + row + x];
-  V b_m = an_m, b_c = an_c, pih = an_c;
-  for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
-    const long long base = (long long)wz(za) * P;
-    const V an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
-    const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
-    const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
-    V p = *(const V *)&pi[base + row + x];
-    V f1;
-    auto stage_a = [&](auto div) {
-#pragma unroll
-      for (int j = 0; j < W; j++) {
-        const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
-        const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
-        T pj = p[j];
-        if constexpr (sizeof(T) == 8)
-          site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], an_p[j],
-                                                      an_m[j], an_c[j], pj, dk0);
-        else
-          site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
-        p[j] = pj;
-        f1[j] = an_c[j] + pj * dd0;
-      }
-    };
-    if constexpr (sizeof(T) == 8) {
-      bool ok = true;
-#pragma unroll
-      for (int j = 0; j < W; j++) ok = ok && fast_div_site(an_c[j]);
-      with_div_mode(k, ok, stage_a);
-    } else {
-      stage_a(std::integral_constant<int, DIV_IEEE>());
-    }
-    T *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
-    *(V *)&buf[ty * N + x] = f1;
-    if (za >= z0 + 1 && inner) {
-      const int zb = za - 1;
-      const T *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
-      const T bxp = bb[ty * N + xp], bxm = bb[ty * N + xm];
-      const V byp = *(const V *)&bb[(ty + 1) * N + x], bym = *(const V *)&bb[(ty - 1) * N + x];
-      V q = pih, f2;
-      auto stage_b = [&](auto div) {
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-          const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
-          const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
-          T qj = q[j];
-          if constexpr (sizeof(T) == 8)
-            site_update_f64<POW2, decltype(div)::value>(k, right, left, byp[j], bym[j], f1[j],
-                                                        b_m[j], b_c[j], qj, dk1);
-          else
-            site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
-          q[j] = qj;
-          f2[j] = b_c[j] + qj * dd1;
-        }
-      };
-      if constexpr (sizeof(T) == 8) {
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < W; j++) ok = ok && fast_div_site(b_c[j]);
-        with_div_mode(k, ok, stage_b);
-      } else {
-        stage_b(std::integral_constant<int, DIV_IEEE>());
-      }
-      const long long bbase = (long long)zb * P;
-      __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
-      __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
-    }
-    // The only barrier of the iteration.  LDS holds two planes: stage A of
-    // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
-    // written one iteration earlier.  This barrier orders that write before
-    // the next iteration's stage B reads it, and this iteration's stage B
-    // reads before the next stage A overwrites the buffer they use.
-    __syncthreads();
-    b_m = b_c;
-    b_c = f1;
-    pih = p;
-    an_m = an_c;
-    an_c = an_p;
-  }
+  const int z0 = zt * ZB;
+  step2_tile<T, C, POW2, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+                            (int)threadIdx.y, true, yt * (RT - 2), z0, min(z0 + ZB, N), k, dk0, dd0,
+                            dk1, dd1);
 }
 
 // updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.

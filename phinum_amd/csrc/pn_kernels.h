@@ -125,3 +125,119 @@ struct RedCoef {
   double d6;    // 720.0*Nc*Nc*Kappa*Kappa               (:571)
 };
 
+
+// Two leapfrog steps on one tile of full x rows (temporal blocking), shared by
+// the single-lattice and batched two-step kernels.  Thread (tx, ty) owns W
+// sites of row y0-1+ty; the tile has RT rows, of which the inner RT-2 are
+// output rows.  Stage A advances step n on all RT rows, stage B step n+1 on
+// the inner rows, reading the intermediate Phi of its plane from LDS (two
+// planes of RT*N values) and the planes above and below from registers.  The
+// z loop runs stage A one plane ahead of stage B, plus one extra plane at each
+// end of [z0, z1).  Halo rows and planes are recomputed by both neighbouring
+// tiles with the same arithmetic, so the result is bit-identical to two single
+// steps.  Pi is read from `pi` and written to `pio`: a neighbouring tile's
+// stage A still needs the old Pi of the rows this tile writes.  Threads with
+// active == false only join the barriers (one per plane), which every thread
+// of the block must reach.
+template <typename T, typename C, bool POW2, int W>
+__device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *__restrict__ pi,
+                                           T *__restrict__ pio, T *__restrict__ phin, T *lds, int N,
+                                           int RT, int tx, int ty, bool active, int y0, int z0,
+                                           int z1, const C &k, T dk0, T dd0, T dk1, T dd1) {
+  typedef typename VecT<T, W>::type V;
+  const int TY = RT - 2;
+  const int x = W * tx;
+  int y = y0 - 1 + ty;
+  y = y < 0 ? y + N : (y >= N ? y - N : y);
+  const bool inner = active && ty >= 1 && ty <= TY && (y0 + ty - 1) < N;
+  const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
+  V an_m = {}, an_c = {};
+  if (active) {
+    an_m = *(const V *)&phi[wz(z0 - 2) * P + row + x];
+    an_c = *(const V *)&phi[wz(z0 - 1) * P + row + x];
+  }
+  V b_m = an_m, b_c = an_c, pih = an_c;
+  for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
+    V an_p = {}, p = {}, f1 = {};
+    if (active) {
+      const long long base = (long long)wz(za) * P;
+      an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
+      const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
+      const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
+      p = *(const V *)&pi[base + row + x];
+      auto stage_a = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
+          const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
+          T pj = p[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], an_p[j],
+                                                        an_m[j], an_c[j], pj, dk0);
+          else
+            site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
+          p[j] = pj;
+          f1[j] = an_c[j] + pj * dd0;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(an_c[j]);
+        with_div_mode(k, ok, stage_a);
+      } else {
+        stage_a(std::integral_constant<int, DIV_IEEE>());
+      }
+      T *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
+      *(V *)&buf[ty * N + x] = f1;
+    }
+    if (za >= z0 + 1 && inner) {
+      const int zb = za - 1;
+      const T *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
+      const T bxp = bb[ty * N + xp], bxm = bb[ty * N + xm];
+      const V byp = *(const V *)&bb[(ty + 1) * N + x], bym = *(const V *)&bb[(ty - 1) * N + x];
+      V q = pih, f2;
+      auto stage_b = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
+          const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
+          T qj = q[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, byp[j], bym[j], f1[j],
+                                                        b_m[j], b_c[j], qj, dk1);
+          else
+            site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
+          q[j] = qj;
+          f2[j] = b_c[j] + qj * dd1;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(b_c[j]);
+        with_div_mode(k, ok, stage_b);
+      } else {
+        stage_b(std::integral_constant<int, DIV_IEEE>());
+      }
+      const long long bbase = (long long)zb * P;
+      __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
+      __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
+    }
+    // The only barrier of the iteration.  LDS holds two planes: stage A of
+    // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
+    // written one iteration earlier.  This barrier orders that write before
+    // the next iteration's stage B reads it, and this iteration's stage B
+    // reads before the next stage A overwrites the buffer they use.
+    __syncthreads();
+    b_m = b_c;
+    b_c = f1;
+    pih = p;
+    an_m = an_c;
+    an_c = an_p;
+  }
+}

@@ -23,14 +23,16 @@ def _init(N, seed):
     return 2.449 + 0.3 * rng.standard_normal(N ** 3), 0.1 * rng.standard_normal(N ** 3)
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_bitwise_equals_oracle(use_graph):
+@pytest.mark.parametrize("use_graph,two_step", [(True, True), (False, True), (True, False),
+                                                (False, False)])
+def test_batch_bitwise_equals_oracle(use_graph, two_step):
     sizes = [8, 16, 12, 24, 8, 20]
     gs = [0.1, 0.2, 0.3, 0.4, 0.25, 0.15]
     reals = _reals(sizes, gs)
-    steps = 230  # two graph replays plus a remainder
+    steps = 231  # two graph replays plus an odd remainder
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, steps)
     with pn.Batch(reals) as B:
+        assert B.two_step(two_step) == two_step
         init = [_init(N, 100 + i) for i, N in enumerate(sizes)]
         for i in range(len(sizes)):
             B.view(i).set_fields(*init[i])
@@ -116,14 +118,16 @@ def test_decayed_realization_is_frozen():
     assert np.array_equal(f1, ophi)
 
 
-def test_views_analysis_matches_single_context():
+@pytest.mark.parametrize("steps", [20, 22])
+def test_views_analysis_matches_single_context(steps):
+    # 22 steps = 11 two-step launches: Pi ends in the second buffer
     sizes = [16, 32]
     reals = _reals(sizes, [0.2, 0.35])
     with pn.Batch(reals) as B:
         for i, N in enumerate(sizes):
             v = B.view(i)
             v.initial_conditions(pn.ic_params(10 + i, a=0.5, Lambda=6.0, phi0=np.sqrt(6.0)))
-        dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, 20)
+        dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, steps)
         B.evolve(dtk, dtd, use_graph=False)
         for i, N in enumerate(sizes):
             r = reals[i]
@@ -131,7 +135,8 @@ def test_views_analysis_matches_single_context():
             with pn.Lattice(N, 0.5, 1, 1.0, -1, r["kappa"]) as L:
                 L.initial_conditions(pn.ic_params(10 + i, a=0.5, Lambda=6.0, phi0=np.sqrt(6.0)))
                 L.evolve(dtk, dtd)
-                assert np.array_equal(v.get_fields()[0], L.get_fields()[0])
+                vf, lf = v.get_fields(), L.get_fields()
+                assert np.array_equal(vf[0], lf[0]) and np.array_equal(vf[1], lf[1])
                 a1, a2 = v.volume_averages().as_array(), L.volume_averages().as_array()
                 assert np.array_equal(a1, a2)
                 s1, s2 = v.spectrum(), L.spectrum()
@@ -149,3 +154,26 @@ def test_view_cannot_step_alone():
         with pytest.raises(pn.PhiNumError) as e:
             v.evolve([0.01], [0.01])
         assert e.value.code == pn.PN_ERR_STATE
+
+
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_batch_two_step_equals_single_steps_with_decay(prec):
+    # mixed sizes, a realization frozen part-way, odd step counts, graph
+    # replays and eager launches: the two modes give the same bits
+    sizes = [8, 16, 12, 24, 64, 20, 36]
+    reals = _reals(sizes, [0.1, 0.2, 0.3, 0.4, 0.25, 0.15, 0.35])
+    init = [_init(N, 300 + i) for i, N in enumerate(sizes)]
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, 333)
+    out = []
+    for two in (True, False):
+        with pn.Batch(reals, precision=prec) as B:
+            assert B.two_step(two) == two
+            for i in range(len(sizes)):
+                B.view(i).set_fields(*init[i])
+            B.evolve(dtk[:151], dtd[:151])
+            B.set_decayed([0, 1, 0, 0, 1, 0, 0])
+            B.evolve(dtk[151:], dtd[151:], use_graph=False)
+            out.append([B.view(i).get_fields() for i in range(len(sizes))])
+    for i in range(len(sizes)):
+        assert np.array_equal(out[0][i][0], out[1][i][0]), i
+        assert np.array_equal(out[0][i][1], out[1][i][1]), i
