@@ -205,6 +205,45 @@ int pn_fft_modes(pn_ctx *c, int field, double *out);
  *         peak = max Delta.
  * Connectivity: 6 neighbours, periodic in x and y, open in z (islands.cpp:200-236). */
 int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out);
+/* Island number (row of the last pn_islands catalog) of each site pos[i]
+ * (reference pos() index of an interior site), or -1 outside the mask.
+ * Replaces the b[] array lookups of Full_Analysis (islands.cpp:798-817). */
+int pn_islands_lookup(pn_ctx *c, int n, const long long *pos, int *island);
+
+/* ---- island tracking over time (islands.cpp:61-70,651-736,788-835) ------
+ * A tracker keeps the reference's found_islands list across Full_Analysis
+ * calls: islands are re-grown from their last peak in order of decreasing
+ * volume (merges drop the later one), retired when their peak leaves the
+ * mask, and new components get new numbers.  Histories keep the first two
+ * and the latest three entries (islands.cpp:255-267). */
+typedef struct pn_tracker pn_tracker;
+#define PN_TRACK_HIST 5
+typedef struct pn_track_entry {
+  double at_time;       /* tau of the analysis                             */
+  long long center_pos; /* pos() index of the peak                         */
+  double peak_value, avg_phi, volume, pot_en, kin_en, grad_en;
+} pn_track_entry;
+typedef struct pn_tracked_island {
+  long long index;      /* number given at birth (number_of_islands)       */
+  int status;           /* 0 island, 1 expanding bubble, 2 oscillon        */
+  int disp;             /* 1: this process holds the island's peak         */
+  int nhist;            /* entries in hist                                 */
+  pn_track_entry hist[PN_TRACK_HIST];
+} pn_tracked_island;
+int pn_tracker_create(pn_tracker **out);
+int pn_tracker_destroy(pn_tracker *t);
+const char *pn_tracker_last_error(const pn_tracker *t);
+/* One Full_Analysis call on the context's current fields at threshold thr
+ * (mode as pn_islands; the caller supplies thr, e.g. 4<E> for mode 1 as in
+ * islands.cpp:783-786) stamped with time tau.  outdir != NULL appends the
+ * reference's non_critical_islands.txt, islands.txt and expanding_bubbles.txt
+ * rows there (islands.cpp:651-736). */
+int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
+                      const char *outdir);
+int pn_tracker_count(const pn_tracker *t, int *n);
+/* i-th tracked island in list order (after the update: the survivors in
+ * decreasing volume of their previous entry, then the new ones). */
+int pn_tracker_get(const pn_tracker *t, int i, pn_tracked_island *out);
 
 /* Energy density per site rho = pi^2/2 + (1/2) sum_i (grad+_i phi)^2 + V(phi),
  * the summand of Energy (fieldanalysis.cpp:625), dense order. */

@@ -6,13 +6,21 @@
 //
 // Usage: ref_harness <in.bin> <out.bin> -Nt N -at a -M m2 -Mmode s -L Lambda
 //                    -Lmode s -Kappa k -T0 t0 -dt base
-// in.bin : int32 nsteps, int32 islands (0/1), double thr (<0: 2.57583*sqrt(<phi^2>)),
+// in.bin : int32 nsteps, int32 islands, double thr (<0: 2.57583*sqrt(<phi^2>)),
 //          double phi[N^3], double pi[N^3]        (x fastest, interior only)
 // out.bin: double phi[N^3], pi[N^3], avg[10], ext[3], int32 extpos[3],
 //          double thr, int32 nisl, double cat[nisl*10], int32 labels[N^3]
+// islands 0/1: nsteps steps, then (1) one Full_Analysis.  islands R >= 2
+// (tracking): Full_Analysis at the start and after each of R-1 blocks of
+// nsteps steps, the reference writing its island files into ./output/0/;
+// out.bin then has nisl = 0, labels -1 and, appended, int32 ntrack and per
+// tracked island int64 index, int32 status, disp, nhist and nhist x 8 doubles
+// (at_time, peak pos, Peak_value, avPhi, V, PotEn, KinEn, GradEn).
 #define MPICH_SKIP_MPICXX 1
 #include "islands.cpp"
 #include "renormalization.cpp"
+#include <sys/stat.h>
+
 #include <cstdio>
 #include <vector>
 
@@ -49,7 +57,22 @@ int main(int argc, char **argv) {
   Communication::exchange();
 
   reset();
-  for (int s = 0; s < nsteps; s++) evolve();
+  const bool tracking = islands >= 2;
+  if (tracking) {
+    mkdir("output", 0755);
+    mkdir("output/0", 0755);  // LASTRUN is 0
+    island_analysis::init();
+    CalculateVolumeAverages();
+    island_analysis::thres = thr >= 0 ? thr : 2.57583 * sqrt(LocalLattice::avgphi2);
+    thr = island_analysis::thres;
+    for (int r = 0; r < islands; r++) {
+      if (r > 0)
+        for (int s = 0; s < nsteps; s++) evolve();
+      Full_Analysis();
+    }
+  } else {
+    for (int s = 0; s < nsteps; s++) evolve();
+  }
 
   CalculateVolumeAverages();
   CalculateExtrema();
@@ -76,7 +99,7 @@ int main(int argc, char **argv) {
   int nisl = 0;
   std::vector<double> cat;
   std::vector<int> lab(n3, -1);
-  if (islands) {
+  if (islands == 1) {
     island_analysis::init();
     island_analysis::thres = thr >= 0 ? thr : 2.57583 * sqrt(LocalLattice::avgphi2);
     thr = island_analysis::thres;
@@ -97,6 +120,22 @@ int main(int argc, char **argv) {
   std::fwrite(&nisl, 4, 1, fo);
   if (nisl) std::fwrite(cat.data(), 8, cat.size(), fo);
   std::fwrite(lab.data(), 4, n3, fo);
+  if (tracking) {
+    std::vector<island> &F = island_analysis::found_islands;
+    const int ntrack = (int)F.size();
+    std::fwrite(&ntrack, 4, 1, fo);
+    for (int i = 0; i < ntrack; i++) {
+      const long long index = F[i].index;
+      const int hdr[3] = {F[i].status, (int)F[i].disp, (int)F[i].V.size()};
+      std::fwrite(&index, 8, 1, fo);
+      std::fwrite(hdr, 4, 3, fo);
+      for (size_t j = 0; j < F[i].V.size(); j++) {
+        const double e[8] = {F[i].at_time[j], (double)F[i].center[j].pos, F[i].Peak_value[j],
+                             F[i].avPhi[j], F[i].V[j], F[i].PotEn[j], F[i].KinEn[j], F[i].GradEn[j]};
+        std::fwrite(e, 8, 8, fo);
+      }
+    }
+  }
   std::fclose(fo);
   MPI_Finalize();
   return 0;

@@ -86,6 +86,17 @@ class PnCatalog(C.Structure):
                 ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
 
 
+class PnTrackEntry(C.Structure):
+    _fields_ = [("at_time", C.c_double), ("center_pos", C.c_longlong), ("peak_value", C.c_double),
+                ("avg_phi", C.c_double), ("volume", C.c_double), ("pot_en", C.c_double),
+                ("kin_en", C.c_double), ("grad_en", C.c_double)]
+
+
+class PnTrackedIsland(C.Structure):
+    _fields_ = [("index", C.c_longlong), ("status", C.c_int), ("disp", C.c_int), ("nhist", C.c_int),
+                ("hist", PnTrackEntry * 5)]
+
+
 _lib = None
 
 
@@ -118,6 +129,14 @@ def lib():
         "pn_spectrum_get": (C.c_int, [C.c_void_p, P(PnSpectrum)]),
         "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pn_islands": (C.c_int, [C.c_void_p, C.c_int, C.c_double, P(PnCatalog)]),
+        "pn_islands_lookup": (C.c_int, [C.c_void_p, C.c_int, P(C.c_longlong), P(C.c_int)]),
+        "pn_tracker_create": (C.c_int, [P(C.c_void_p)]),
+        "pn_tracker_destroy": (C.c_int, [C.c_void_p]),
+        "pn_tracker_last_error": (C.c_char_p, [C.c_void_p]),
+        "pn_tracker_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                        C.c_char_p]),
+        "pn_tracker_count": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "pn_tracker_get": (C.c_int, [C.c_void_p, C.c_int, P(PnTrackedIsland)]),
         "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
         "pn_ic_modes": (C.c_int, [C.c_int, C.c_double, P(PnIcParams), dp, dp]),
         "pn_initial_conditions": (C.c_int, [C.c_void_p, P(PnIcParams)]),
@@ -361,6 +380,58 @@ def time_factors(T0, dtbase, a, nsteps):
         last_f = tau
         taus[n] = tau
     return dtk, dtd, taus
+
+
+class Tracker:
+    """Island tracking over time (Full_Analysis called repeatedly,
+    islands.cpp:788-835) on the GPU catalog."""
+
+    STATUS = {0: "island", 1: "expanding bubble", 2: "oscillon"}
+
+    def __init__(self):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.pn_tracker_create(C.byref(h))
+        if rc != PN_OK:
+            raise PhiNumError(rc, "pn_tracker_create failed")
+        self.h = h
+
+    def update(self, lattice, mode, thr, tau, outdir=None):
+        rc = self.L.pn_tracker_update(self.h, lattice.h, int(mode), float(thr), float(tau),
+                                      outdir.encode() if outdir else None)
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_tracker_last_error(self.h).decode())
+
+    def islands(self):
+        """[(index, status, disp, [(at_time, peak pos, peak value, avg phi, volume, PotEn,
+        KinEn, GradEn), ...])] in list order."""
+        n = C.c_int()
+        self.L.pn_tracker_count(self.h, C.byref(n))
+        out = []
+        for i in range(n.value):
+            t = PnTrackedIsland()
+            self.L.pn_tracker_get(self.h, i, C.byref(t))
+            hist = [(e.at_time, int(e.center_pos), e.peak_value, e.avg_phi, e.volume, e.pot_en,
+                     e.kin_en, e.grad_en) for e in t.hist[:t.nhist]]
+            out.append((int(t.index), int(t.status), int(t.disp), hist))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pn_tracker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Batch:

@@ -486,3 +486,44 @@ extern "C" int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out) {
   if (c->p.precision == PN_FP64) return run_islands<double>(c, k, out);
   return run_islands<float>(c, k, out);
 }
+
+// island number of the last pn_islands call at given sites (or -1 outside the
+// mask): label[] holds each site's root, cid[] the number stored at roots
+__global__ void k_isl_lookup(const int *__restrict__ label, const int *__restrict__ cid,
+                             const long long *__restrict__ dense, int n, int *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = label[dense[i]];
+  out[i] = r >= 0 ? cid[r] : -1;
+}
+
+extern "C" int pn_islands_lookup(pn_ctx *c, int n, const long long *pos, int *island) {
+  if (!c || n < 0 || (n > 0 && (!pos || !island)))
+    return pn_fail(c, PN_ERR_ARG, "pn_islands_lookup: bad argument");
+  if (!c->isl) return pn_fail(c, PN_ERR_STATE, "pn_islands_lookup: no pn_islands call yet");
+  if (n == 0) return PN_OK;
+  const long long P = (long long)c->p.N * c->p.N;
+  std::vector<long long> dense(n);
+  for (int i = 0; i < n; i++) {
+    dense[i] = pos[i] - P;  // reference pos() (one z halo plane below) -> dense
+    if (dense[i] < 0 || dense[i] >= c->n3)
+      return pn_fail(c, PN_ERR_ARG, "pn_islands_lookup: pos %lld is not an interior site", pos[i]);
+  }
+  PN_HIP(c, hipSetDevice(c->p.device));
+  long long *dd = nullptr;
+  int *dout = nullptr;
+  PN_HIP(c, hipMalloc(&dd, sizeof(long long) * n));
+  hipError_t e = hipMalloc(&dout, sizeof(int) * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(dd, dense.data(), sizeof(long long) * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_isl_lookup, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->isl->label,
+                       c->isl->cid, dd, n, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(island, dout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(dd);
+  hipFree(dout);
+  if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands_lookup: %s", hipGetErrorString(e));
+  return PN_OK;
+}

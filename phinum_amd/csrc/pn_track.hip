@@ -1,0 +1,230 @@
+// pn_track.hip -- island tracking over time: the bookkeeping of Full_Analysis
+// (islands.cpp:788-835), Old_islands (:651-705) and print_bubble_history
+// (:707-736) on top of the GPU island catalog (pn_islands).
+//
+// Each update is one Full_Analysis call.  The tracked islands are sorted by
+// their latest volume, largest first, with std::sort and the reference's
+// comparison (islands.cpp:29-32), so islands of equal volume come out in the
+// same order as in the reference.  An island whose last peak site left the
+// mask is retired (written to non_critical_islands.txt and islands.txt).  The
+// survivors, in sorted order, re-grow from their last peak: the first to reach
+// a component claims it and records a new history entry, a later one whose
+// peak lies in an already claimed component has merged and is dropped.  The
+// components nobody claimed are new islands, numbered in raster order of
+// their first site.  The reference re-grows by a BFS from the old peak; here
+// the component containing that site comes from the GPU labelling
+// (pn_islands_lookup) and its statistics from the catalog, which gives the
+// same partition and peak (a unique maximum does not depend on the seed) and
+// the same sums up to summation order.
+//
+// Histories keep at most five entries: the first two and the latest three
+// (islands.cpp:255-267).  Text outputs use the reference's stream formats.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pn_internal.h"
+
+struct pn_tracker {
+  struct Isl {
+    long long index = 0;
+    int disp = 1;
+    int status = 0;
+    std::vector<pn_track_entry> h;
+  };
+  std::vector<Isl> found;
+  long long number_of_islands = 0;
+  std::string err;
+};
+
+static int tfail(pn_tracker *t, int code, const std::string &m) {
+  if (t) t->err = m;
+  pn_set_thread_error(m.c_str());
+  return code;
+}
+
+extern "C" int pn_tracker_create(pn_tracker **out) {
+  if (!out) return tfail(nullptr, PN_ERR_ARG, "pn_tracker_create: NULL argument");
+  *out = new (std::nothrow) pn_tracker();
+  return *out ? PN_OK : tfail(nullptr, PN_ERR_NOMEM, "pn_tracker_create: out of host memory");
+}
+
+extern "C" int pn_tracker_destroy(pn_tracker *t) {
+  delete t;
+  return PN_OK;
+}
+
+extern "C" const char *pn_tracker_last_error(const pn_tracker *t) {
+  return t ? t->err.c_str() : pn_last_error(nullptr);
+}
+
+extern "C" int pn_tracker_count(const pn_tracker *t, int *n) {
+  if (!t || !n) return tfail(const_cast<pn_tracker *>(t), PN_ERR_ARG, "pn_tracker_count: NULL argument");
+  *n = (int)t->found.size();
+  return PN_OK;
+}
+
+extern "C" int pn_tracker_get(const pn_tracker *t, int i, pn_tracked_island *out) {
+  if (!t || !out || i < 0 || i >= (int)t->found.size())
+    return tfail(const_cast<pn_tracker *>(t), PN_ERR_ARG, "pn_tracker_get: bad argument");
+  const pn_tracker::Isl &s = t->found[i];
+  out->index = s.index;
+  out->status = s.status;
+  out->disp = s.disp;
+  out->nhist = (int)s.h.size();
+  for (int j = 0; j < out->nhist; j++) out->hist[j] = s.h[j];
+  return PN_OK;
+}
+
+namespace {
+
+// point::x/y/z (fieldanalysis.cpp:19-32) for one process (id 0)
+struct Coords {
+  long long N;
+  long long x(long long pos) const { return (pos % (N * N)) % N; }
+  long long y(long long pos) const { return (pos % (N * N)) / N; }
+  long long z(long long pos) const { return -1 + pos / (N * N); }
+};
+
+void write_entry(std::ofstream &o, const pn_track_entry &e, long long index, const Coords &c) {
+  o << e.at_time << ' ' << c.x(e.center_pos) << ' ' << c.y(e.center_pos) << ' ' << c.z(e.center_pos)
+    << "        ";
+  o << e.peak_value << ' ' << e.avg_phi << ' ' << e.volume << ' ';
+  o << e.grad_en << ' ' << e.pot_en << ' ' << e.kin_en << ' ' << index << std::endl;
+}
+
+// the islands.txt row (islands.cpp:683,717)
+void write_init_row(std::ofstream &o, const pn_tracker::Isl &s, int status, double thres) {
+  const pn_track_entry &e0 = s.h[0];
+  o << std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.) << ' ' << e0.peak_value << ' '
+    << e0.at_time << ' ' << s.index << ' ' << status << ' '
+    << std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.) /
+           std::sqrt(std::log(e0.peak_value * e0.peak_value / thres / thres))
+    << std::endl;
+}
+
+}  // namespace
+
+extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
+                                 const char *outdir) {
+  if (!t || !c) return tfail(t, PN_ERR_ARG, "pn_tracker_update: NULL argument");
+  // the current components
+  std::vector<pn_island> cat(1024);
+  pn_catalog pc{(int)cat.size(), 0, 0.0, cat.data(), nullptr};
+  int rc = pn_islands(c, mode, thr, &pc);
+  if (rc == PN_ERR_CAPACITY) {
+    cat.resize(pc.count);
+    pc.capacity = pc.count;
+    pc.islands = cat.data();
+    rc = pn_islands(c, mode, thr, &pc);
+  }
+  if (rc != PN_OK) return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
+  cat.resize(pc.count);
+
+  // largest latest volume first (islands.cpp:29-32,797); the comparison and
+  // the algorithm are the reference's, so ties keep its order
+  std::sort(t->found.begin(), t->found.end(), [](const pn_tracker::Isl &b1, const pn_tracker::Isl &b2) {
+    return b1.h.back().volume > b2.h.back().volume;
+  });
+
+  // component of each tracked island's last peak (-1: the site left the mask)
+  const int n = (int)t->found.size();
+  std::vector<long long> peaks(n);
+  std::vector<int> comp(n, -1);
+  for (int i = 0; i < n; i++) peaks[i] = t->found[i].h.back().center_pos;
+  if (n > 0 && (rc = pn_islands_lookup(c, n, peaks.data(), comp.data())) != PN_OK)
+    return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
+
+  const Coords co{c->p.N};
+  std::ofstream safe, init, bubbles;
+  const bool files = outdir != nullptr;
+  if (files) {
+    // Old_islands opens (and so creates) these two on every call (:663-664)
+    safe.open(std::string(outdir) + "/non_critical_islands.txt", std::ios::app);
+    init.open(std::string(outdir) + "/islands.txt", std::ios::app);
+    if (!safe || !init) return tfail(t, PN_ERR_ARG, std::string("pn_tracker_update: cannot write to ") + outdir);
+  }
+
+  // Old_islands: retire the islands whose peak site is no longer in the mask
+  std::vector<pn_tracker::Isl> alive;
+  std::vector<int> acomp;
+  for (int i = 0; i < n; i++) {
+    pn_tracker::Isl &s = t->found[i];
+    if (comp[i] >= 0) {
+      alive.push_back(std::move(s));
+      acomp.push_back(comp[i]);
+      continue;
+    }
+    if (files) {
+      if (s.disp) {
+        for (const pn_track_entry &e : s.h) write_entry(safe, e, s.index, co);
+        safe << std::endl;
+      }
+      write_init_row(init, s, s.status, thr);
+    }
+  }
+
+  // survivors re-grow in sorted order; a claimed component means a merge
+  std::vector<char> claimed(cat.size(), 0);
+  std::vector<pn_tracker::Isl> next;
+  for (size_t i = 0; i < alive.size(); i++) {
+    const int k = acomp[i];
+    if (claimed[k]) continue;  // merged into an island earlier in the order
+    claimed[k] = 1;
+    pn_tracker::Isl s = std::move(alive[i]);
+    const pn_island &I = cat[k];
+    pn_track_entry e;
+    e.at_time = tau;
+    e.center_pos = I.peak_pos;
+    e.peak_value = I.peak_value;
+    e.avg_phi = I.avg_phi;
+    e.volume = I.volume;
+    e.pot_en = I.pot_en;
+    e.kin_en = I.kin_en;
+    e.grad_en = I.grad_en;
+    s.h.push_back(e);
+    if (s.h.size() > PN_TRACK_HIST) s.h.erase(s.h.begin() + 2);
+    s.status = 0;
+    if (s.h.back().grad_en + s.h.back().pot_en < 0.) {  // is_bubble (islands.cpp:61-65)
+      if (files) {
+        // print_bubble_history (:707-736) opens expanding_bubbles.txt when called
+        if (!bubbles.is_open()) bubbles.open(std::string(outdir) + "/expanding_bubbles.txt", std::ios::app);
+        if (s.disp) write_init_row(init, s, 1, thr);
+        for (size_t j = 1; j < s.h.size(); j++) write_entry(bubbles, s.h[j], s.index, co);
+        bubbles << std::endl;
+      }
+      s.status = 1;
+    }
+    if (s.status == 0 && s.h.back().at_time - s.h.front().at_time > 40.)  // is_oscillon (:67-70)
+      s.status = 2;
+    next.push_back(std::move(s));
+  }
+
+  // new islands: unclaimed components in raster order of their first site
+  for (size_t k = 0; k < cat.size(); k++) {
+    if (claimed[k]) continue;
+    const pn_island &I = cat[k];
+    pn_tracker::Isl s;
+    s.index = t->number_of_islands++;
+    s.disp = 1;
+    s.status = 0;
+    pn_track_entry e;
+    e.at_time = tau;
+    e.center_pos = I.peak_pos;
+    e.peak_value = I.peak_value;
+    e.avg_phi = I.avg_phi;
+    e.volume = I.volume;
+    e.pot_en = I.pot_en;
+    e.kin_en = I.kin_en;
+    e.grad_en = I.grad_en;
+    s.h.push_back(e);
+    next.push_back(std::move(s));
+  }
+  t->found = std::move(next);
+  return PN_OK;
+}

@@ -127,6 +127,18 @@ def evolve(m, phi_dense, pi_dense, nsteps, T0=0.0, dtbase=0.1):
     return from_ref(phi, N), from_ref(pi, N)
 
 
+def evolve_tables(m, phi_dense, pi_dense, dtk, dtd):
+    """evolve with given per-step factors (a slice of one continuous replay)."""
+    N = m["N"]
+    phi = to_ref(phi_dense, N)
+    pi = to_ref(pi_dense, N)
+    dtk = np.ascontiguousarray(dtk, dtype=np.float64)
+    dtd = np.ascontiguousarray(dtd, dtype=np.float64)
+    p = params(m)
+    lib().or_evolve(C.byref(p), dptr(phi), dptr(pi), int(dtk.size), dptr(dtk), dptr(dtd))
+    return from_ref(phi, N), from_ref(pi, N)
+
+
 def volume_averages(m, phi_dense, pi_dense, want_delta=False):
     N = m["N"]
     phi = to_ref(phi_dense, N)
@@ -257,7 +269,7 @@ def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtb
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            f.write(struct.pack("<iid", nsteps, 1 if islands else 0, thr))
+            f.write(struct.pack("<iid", nsteps, int(islands), thr))
             f.write(np.ascontiguousarray(phi_dense, dtype=np.float64).tobytes())
             f.write(np.ascontiguousarray(pi_dense, dtype=np.float64).tobytes())
         args = [REF_HARNESS_FAST if fast else REF_HARNESS, fin, fout,
@@ -269,6 +281,11 @@ def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtb
         if r.returncode != 0:
             raise RuntimeError("ref harness failed: %s %s" % (r.stdout, r.stderr))
         b = open(fout, "rb").read()
+        files = {}
+        odir = os.path.join(td, "output", "0")
+        if os.path.isdir(odir):
+            for fn in sorted(os.listdir(odir)):
+                files[fn] = open(os.path.join(odir, fn)).read()
     n3 = N ** 3
     o = 0
     phi = np.frombuffer(b, np.float64, n3, o).copy(); o += 8 * n3
@@ -279,9 +296,19 @@ def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtb
     thr_used = struct.unpack_from("<d", b, o)[0]; o += 8
     nisl = struct.unpack_from("<i", b, o)[0]; o += 4
     cat = np.frombuffer(b, np.float64, 10 * nisl, o).reshape(nisl, 10).copy(); o += 80 * nisl
-    labels = np.frombuffer(b, np.int32, n3, o).copy()
+    labels = np.frombuffer(b, np.int32, n3, o).copy(); o += 4 * n3
+    track = None
+    if int(islands) >= 2:
+        nt = struct.unpack_from("<i", b, o)[0]; o += 4
+        track = []
+        for _ in range(nt):
+            index = struct.unpack_from("<q", b, o)[0]; o += 8
+            status, disp, nh = struct.unpack_from("<3i", b, o); o += 12
+            e = np.frombuffer(b, np.float64, 8 * nh, o).reshape(nh, 8); o += 64 * nh
+            hist = [(r[0], int(r[1]), r[2], r[3], r[4], r[5], r[6], r[7]) for r in e]
+            track.append((index, status, disp, hist))
     return dict(phi=phi, pi=pi, avg=avg, ext=ext, extpos=extpos, thr=thr_used, cat=cat,
-                labels=labels)
+                labels=labels, track=track, files=files)
 
 
 # ---------------------------------------------------------------------------
@@ -350,3 +377,121 @@ def adj_reverse(p, phi0, pi0, dtk, dtd):
                               dptr(np.ascontiguousarray(dtk)), dptr(np.ascontiguousarray(dtd)),
                               dptr(gphi), dptr(gpi), dptr(gc))
     return F, gphi, gpi, gc
+
+
+# ---------------------------------------------------------------------------
+# island tracking checker (oracle/oracle_track.cpp)
+# ---------------------------------------------------------------------------
+ORACLE_TRACK_LIB = os.path.join(ORACLE_DIR, "liboracle_track.so")
+
+
+class TrackEntry(C.Structure):
+    _fields_ = [("at_time", C.c_double), ("center_pos", C.c_longlong), ("peak_value", C.c_double),
+                ("avg_phi", C.c_double), ("volume", C.c_double), ("pot_en", C.c_double),
+                ("kin_en", C.c_double), ("grad_en", C.c_double)]
+
+
+class TrackedIsland(C.Structure):
+    _fields_ = [("index", C.c_longlong), ("status", C.c_int), ("disp", C.c_int), ("nhist", C.c_int),
+                ("hist", TrackEntry * 5)]
+
+
+_track_lib = None
+
+
+def _tlib():
+    global _track_lib
+    if _track_lib is None:
+        if not os.path.exists(ORACLE_TRACK_LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR, "liboracle_track.so"], check=True,
+                           capture_output=True)
+        L = C.CDLL(ORACLE_TRACK_LIB)
+        dp = C.POINTER(C.c_double)
+        L.or_track_create.restype = C.c_void_p
+        L.or_track_create.argtypes = [C.POINTER(OrParams)]
+        L.or_track_destroy.argtypes = [C.c_void_p]
+        L.or_track_count.restype = C.c_int
+        L.or_track_count.argtypes = [C.c_void_p]
+        L.or_track_get.argtypes = [C.c_void_p, C.c_int, C.POINTER(TrackedIsland)]
+        L.or_track_update.argtypes = [C.c_void_p, C.c_int, C.c_double, dp, dp, dp, C.c_double,
+                                      C.c_char_p]
+        _track_lib = L
+    return _track_lib
+
+
+def tracked_list(get, n):
+    """[(index, status, disp, [entry tuples])] from a getter filling TrackedIsland."""
+    out = []
+    for i in range(n):
+        t = TrackedIsland()
+        get(i, t)
+        hist = [(e.at_time, int(e.center_pos), e.peak_value, e.avg_phi, e.volume, e.pot_en,
+                 e.kin_en, e.grad_en) for e in t.hist[:t.nhist]]
+        out.append((int(t.index), int(t.status), int(t.disp), hist))
+    return out
+
+
+class Tracker:
+    """Full_Analysis called repeatedly on the CPU (islands.cpp:788-835)."""
+
+    def __init__(self, m):
+        self.m = dict(m)
+        self._p = params(m)
+        self.h = _tlib().or_track_create(C.byref(self._p))
+
+    def update(self, mode, thr, phi_dense, pi_dense, tau, outdir=None):
+        N = self.m["N"]
+        phi = to_ref(phi_dense, N)
+        pi = to_ref(pi_dense, N)
+        delta = None
+        if mode == 1:
+            _, dd = volume_averages(self.m, phi_dense, pi_dense, want_delta=True)
+            delta = to_ref(dd, N)
+        _tlib().or_track_update(self.h, mode, thr, dptr(phi), dptr(pi),
+                                dptr(delta) if delta is not None else None, tau,
+                                outdir.encode() if outdir else None)
+
+    def islands(self):
+        L = _tlib()
+        return tracked_list(lambda i, t: L.or_track_get(self.h, i, C.byref(t)),
+                            L.or_track_count(self.h))
+
+    def close(self):
+        if self.h:
+            _tlib().or_track_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def track_replay(m, phi, pi, analyses, steps, thr, mode=0):
+    """The tracking run of the reference harness on the CPU oracle: an
+    analysis at tau = T0 = 0 and after each block of `steps` steps (one
+    continuous time replay).  Returns (tracked list, {file name: text})."""
+    dtk, dtd, taus = time_factors(steps * (analyses - 1))
+    T = Tracker(m)
+    f, p = np.array(phi, dtype=np.float64), np.array(pi, dtype=np.float64)
+    with tempfile.TemporaryDirectory() as td:
+        for k in range(analyses):
+            if k > 0:
+                sl = slice(steps * (k - 1), steps * k)
+                f, p = evolve_tables(m, f, p, dtk[sl], dtd[sl])
+            T.update(mode, thr, f, p, 0.0 if k == 0 else taus[steps * k - 1], td)
+        files = {fn: open(os.path.join(td, fn)).read() for fn in sorted(os.listdir(td))}
+    out = T.islands()
+    T.close()
+    return out, files
+
+
+def load_track_fixture(path):
+    z = np.load(path)
+    m = dict(N=int(z["N"]), a=float(z["a"]), mass_mode=int(z["mass_mode"]), m2=float(z["m2"]),
+             lambda_mode=int(z["lambda_mode"]), kappa=float(z["kappa"]), Lambda=float(z["Lambda"]))
+    track = []
+    for i in range(z["t_index"].size):
+        nh = int(z["t_nhist"][i])
+        hist = [(r[0], int(r[1]), r[2], r[3], r[4], r[5], r[6], r[7]) for r in z["t_hist"][i, :nh]]
+        track.append((int(z["t_index"][i]), int(z["t_status"][i]), int(z["t_disp"][i]), hist))
+    files = {str(n): str(z["file_" + str(n).replace(".", "_")]) for n in z["file_names"]}
+    return m, z, track, files

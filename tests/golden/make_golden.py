@@ -30,6 +30,59 @@ CASES = [
 ]
 
 
+# island tracking: Full_Analysis at the start and after each block of steps
+TRACK_CASES = [
+    # name, g, N, seed, amp, ball radius, analyses, steps between, thr (<0: 2.57583 sqrt(<phi^2>))
+    ("track_n16_noise", 0.2, 16, 5, 0.9, 0, 6, 5, -1.0),
+    ("track_n20_noise", 0.2, 20, 7, 1.0, 0, 10, 5, -1.0),
+    ("track_n24_bubble", 0.1, 24, 8, 0.5, 6, 10, 100, -1.0),
+    ("track_n16_oscillon", 0.2, 16, 11, 0.9, 0, 50, 20, 0.35 * 6.0 ** 0.5),
+]
+
+
+def track_inputs(N, seed, amp, ball):
+    """Noise around phi = 0 (reference units), optionally a ball at the
+    true-vacuum value 2.979 sqrt(6) of the g = 0.1 potential."""
+    rng = np.random.default_rng(seed)
+    phi = amp * rng.standard_normal(N ** 3)
+    pi = 0.3 * rng.standard_normal(N ** 3)
+    if ball:
+        z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+        r2 = ((x - N / 2) ** 2 + (y - N / 2) ** 2 + (z - N / 2) ** 2).ravel()
+        phi = np.where(r2 < ball ** 2, 2.979 * np.sqrt(6.0), phi)
+    return phi, pi
+
+
+def pack_track(track):
+    """tracked list -> fixed arrays (hist padded with NaN)."""
+    n = len(track)
+    hist = np.full((n, 5, 8), np.nan)
+    for i, (_, _, _, h) in enumerate(track):
+        for j, e in enumerate(h):
+            hist[i, j] = e
+    return dict(t_index=np.array([t[0] for t in track], dtype=np.int64),
+                t_status=np.array([t[1] for t in track], dtype=np.int32),
+                t_disp=np.array([t[2] for t in track], dtype=np.int32),
+                t_nhist=np.array([len(t[3]) for t in track], dtype=np.int32), t_hist=hist)
+
+
+def main_track():
+    d = os.path.join(HERE, "track")
+    os.makedirs(d, exist_ok=True)
+    for name, g, N, seed, amp, ball, R, nst, thr in TRACK_CASES:
+        m = O.model(N, g=g)
+        phi, pi = track_inputs(N, seed, amp, ball)
+        r = O.run_ref(m, phi, pi, nst, islands=R, thr=thr)
+        files = {k: np.array(v) for k, v in r["files"].items()}
+        np.savez_compressed(
+            os.path.join(d, name + ".npz"),
+            N=N, a=m["a"], mass_mode=m["mass_mode"], m2=m["m2"], lambda_mode=m["lambda_mode"],
+            kappa=m["kappa"], Lambda=m["Lambda"], analyses=R, steps=nst, thr=r["thr"],
+            phi_in=phi, pi_in=pi, file_names=np.array(sorted(files)),
+            **{"file_" + k.replace(".", "_"): v for k, v in files.items()}, **pack_track(r["track"]))
+        print("wrote", name, "tracked:", len(r["track"]), "files:", sorted(files))
+
+
 def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
@@ -49,3 +102,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    main_track()
