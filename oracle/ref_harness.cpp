@@ -16,6 +16,8 @@
 // out.bin then has nisl = 0, labels -1 and, appended, int32 ntrack and per
 // tracked island int64 index, int32 status, disp, nhist and nhist x 8 doubles
 // (at_time, peak pos, Peak_value, avPhi, V, PotEn, KinEn, GradEn).
+// With REF_SAVECONFIG=n in the environment the final state is also written by
+// the reference's SaveConfig(n) (./field-n-0.out, ./moment-n.out).
 #define MPICH_SKIP_MPICXX 1
 #include "islands.cpp"
 #include "renormalization.cpp"
@@ -120,6 +122,7 @@ int main(int argc, char **argv) {
   std::fwrite(&nisl, 4, 1, fo);
   if (nisl) std::fwrite(cat.data(), 8, cat.size(), fo);
   std::fwrite(lab.data(), 4, n3, fo);
+  if (const char *sc = std::getenv("REF_SAVECONFIG")) SaveConfig(std::atoi(sc));
   if (tracking) {
     std::vector<island> &F = island_analysis::found_islands;
     const int ntrack = (int)F.size();

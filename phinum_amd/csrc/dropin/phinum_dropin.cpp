@@ -1,6 +1,7 @@
 // phinum_dropin.cpp -- reference-named host layer over the C ABI.
 #include "phinum_dropin.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -33,6 +34,8 @@ int islanddetect = 0;
 int island_mode = 0;
 int island_every = 5;
 int write_output = 1;
+int save_config = -1;
+int extract_config = -1;
 
 static std::string g_err;
 std::string last_error() { return g_err; }
@@ -90,6 +93,8 @@ void GetParameters(int argc, char **argv) {
   gi("islandmode", island_mode);
   gi("islandevery", island_every);
   gi("output", write_output);
+  gi("saveconfig", save_config);
+  gi("extractconfig", extract_config);
   if (Nc != 1) throw std::runtime_error("GetParameters: the device path evolves one component (Nc=1)");
   if (renorm_tadpole != 0)
     throw std::runtime_error("GetParameters: tadpole renormalisation is not implemented upstream either");
@@ -379,22 +384,53 @@ void Print_fp() {  // momentumspace.cpp:280-384
 namespace island_analysis {
 double thres = 1.125;
 int number_of_islands = 0;
-std::vector<pn_island> found_islands;
-void init() { number_of_islands = 0; found_islands.clear(); }
-void fin() { found_islands.clear(); }
+std::vector<pn_tracked_island> found_islands;
+std::vector<pn_island> catalog;
+static pn_tracker *tracker = nullptr;
+void init() {  // islands.cpp:46-51
+  number_of_islands = 0;
+  found_islands.clear();
+  catalog.clear();
+  if (tracker) pn_tracker_destroy(tracker);
+  tracker = nullptr;
+  if (pn_tracker_create(&tracker) != PN_OK) throw std::runtime_error("island_analysis::init: " + last_error());
+}
+void fin() {
+  found_islands.clear();
+  catalog.clear();
+  if (tracker) pn_tracker_destroy(tracker);
+  tracker = nullptr;
+}
 }  // namespace island_analysis
 
-// Catalog of the islands present now (islands.cpp:778-840 on a lattice with
-// no tracked islands): numbering follows the raster scan of :827-835.
+// islands.cpp:778-840: the tracked islands re-grow from their last peaks,
+// retired ones and expanding bubbles go to the run's text files, new
+// components become new islands (pn_tracker_update); `catalog` keeps the
+// components of this call.
 void Full_Analysis() {
   using namespace island_analysis;
   if (island_mode == 1) {  // islands.cpp:783-786
     CalculateVolumeAverages();
     thres = LocalLattice::avgEnergy * 4.;
   }
+  if (!tracker) init();
+  char dir[64];
+  std::snprintf(dir, sizeof dir, "./output/%i", LASTRUN);
+  const bool files = write_output && ID == 0;
+  int rc = pn_tracker_update(tracker, LocalLattice::ctx, island_mode, thres, LocalLattice::tau,
+                             files ? dir : nullptr);
+  if (rc != PN_OK) throw std::runtime_error(std::string("Full_Analysis: ") + pn_tracker_last_error(tracker));
+  int n = 0;
+  pn_tracker_count(tracker, &n);
+  found_islands.resize(n);
+  for (int i = 0; i < n; i++) {
+    pn_tracker_get(tracker, i, &found_islands[i]);
+    number_of_islands = std::max<int>(number_of_islands, (int)found_islands[i].index + 1);
+  }
+  // the components of this call, as numbered by the raster scan
   std::vector<pn_island> buf(4096);
   pn_catalog cat{(int)buf.size(), 0, 0.0, buf.data(), nullptr};
-  int rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
+  rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
   if (rc == PN_ERR_CAPACITY) {
     buf.resize(cat.count);
     cat.capacity = cat.count;
@@ -403,8 +439,50 @@ void Full_Analysis() {
   }
   check(rc, "Full_Analysis");
   buf.resize(cat.count);
-  found_islands = buf;
-  number_of_islands = cat.count;
+  catalog = buf;
+}
+
+// ---- checkpoint text (fieldanalysis.cpp:252-313) ---------------------------
+// ./field-<Name>-<ID>.out: one "Phi Pi" line per interior site in pos()
+// order, %.17e for FP64 fields and %.9e for FP32 ones (the reference's
+// doubleprecision 0 / -1 formats); ./moment-<Name>.out: tau.
+void SaveConfig(int Name) {
+  using namespace LocalLattice;
+  SyncToHost();
+  char fname[256];
+  std::snprintf(fname, sizeof fname, "./field-%i-%i.out", Name, ID);
+  FILE *of = std::fopen(fname, "w");
+  if (!of) throw std::runtime_error(std::string("SaveConfig: cannot write ") + fname);
+  const long end = (long)Nc * N_t * N_t * (N_eta_loc + 1);
+  const char *fmt = precision == 1 ? "%.9e %.9e\n" : "%.17e %.17e\n";
+  for (long K = (long)Nc * N_t * N_t; K < end; K++) std::fprintf(of, fmt, Phi[K], Pi[K]);
+  std::fclose(of);
+  if (ID == 0) {
+    std::snprintf(fname, sizeof fname, "./moment-%i.out", Name);
+    std::ofstream fout(fname);
+    fout << tau << std::endl;
+  }
+}
+
+// Reads a checkpoint back: the fields go to the device, T0 takes the saved
+// tau (the next reset() starts from it).
+void ExtractConfig(int Name) {
+  using namespace LocalLattice;
+  char fname[256];
+  std::snprintf(fname, sizeof fname, "./field-%i-%i.out", Name, ID);
+  FILE *of = std::fopen(fname, "r");
+  if (!of) throw std::runtime_error(std::string("ExtractConfig: cannot read ") + fname);
+  const long end = (long)Nc * N_t * N_t * (N_eta_loc + 1);
+  for (long K = (long)Nc * N_t * N_t; K < end; K++)
+    if (std::fscanf(of, "%lf %lf\n", Phi + K, Pi + K) != 2) {
+      std::fclose(of);
+      throw std::runtime_error(std::string("ExtractConfig: short file ") + fname);
+    }
+  std::fclose(of);
+  SyncToDevice();
+  std::snprintf(fname, sizeof fname, "./moment-%i.out", Name);
+  std::ifstream fin(fname);
+  fin >> T0;
 }
 
 }  // namespace phinum

@@ -44,6 +44,8 @@ extern int islanddetect;      // -islands 0/1 (reference: #define islanddetect)
 extern int island_mode;       // -islandmode 0: phi^2/2+pi^2/2>thr^2/2, 1: Delta>4<E>
 extern int island_every;      // -islandevery n (reference: 5, main.cpp:82)
 extern int write_output;      // -output 0/1: write ./output/<run>/ files
+extern int save_config;       // -saveconfig n: SaveConfig(n) at the end of each run (-1: off)
+extern int extract_config;    // -extractconfig n: ExtractConfig(n) after the initial conditions
 
 void GetParameters(int argc, char **argv);
 void CompleteParameters();
@@ -90,6 +92,8 @@ void Prepare_Output();
 void Print_Info_Console();
 void Print_AveragesExtrema();
 bool false_vacuum_decayed();
+void SaveConfig(int Name);     // fieldanalysis.cpp:252-282
+void ExtractConfig(int Name);  // fieldanalysis.cpp:284-313
 
 namespace MomentumSpace {
 extern double pTmin, pTmax, binsizeT;
@@ -105,7 +109,8 @@ void Print_fp();
 namespace island_analysis {
 extern double thres;
 extern int number_of_islands;
-extern std::vector<pn_island> found_islands;  // last catalog
+extern std::vector<pn_tracked_island> found_islands;  // tracked over time
+extern std::vector<pn_island> catalog;                // components of the last call
 void init();
 void fin();
 }  // namespace island_analysis

@@ -25,6 +25,7 @@ int main(int argc, char **argv) {
       fix_tadpole();
       if (islanddetect) island_analysis::init();
       SetInitialConditions();
+      if (extract_config >= 0) ExtractConfig(extract_config);  // resume from a checkpoint
       if (islanddetect) {  // main.cpp:69-72
         CalculateVolumeAverages();
         island_analysis::thres = 2.57583 * std::sqrt(LocalLattice::avgphi2);
@@ -39,6 +40,7 @@ int main(int argc, char **argv) {
         }
         Print_fp();
       }
+      if (save_config >= 0) SaveConfig(save_config);
       if (islanddetect) island_analysis::fin();
     }
     MomentumSpace::fin();

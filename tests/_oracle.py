@@ -263,7 +263,7 @@ def have_ref(fast=False):
 
 
 def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtbase=0.1,
-            fast=False):
+            fast=False, save_config=None):
     N = m["N"]
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "in.bin")
@@ -277,11 +277,17 @@ def run_ref(m, phi_dense, pi_dense, nsteps, islands=False, thr=-1.0, T0=0.0, dtb
                 "-Mmode", str(m["mass_mode"]), "-L", repr(m.get("Lambda", 6.0)),
                 "-Lmode", str(m["lambda_mode"]), "-Kappa", repr(m["kappa"]),
                 "-T0", repr(T0), "-dt", repr(dtbase)]
-        r = subprocess.run(args, capture_output=True, text=True, cwd=td, timeout=600)
+        env = dict(os.environ)
+        if save_config is not None:
+            env["REF_SAVECONFIG"] = str(int(save_config))
+        r = subprocess.run(args, capture_output=True, text=True, cwd=td, timeout=600, env=env)
         if r.returncode != 0:
             raise RuntimeError("ref harness failed: %s %s" % (r.stdout, r.stderr))
         b = open(fout, "rb").read()
         files = {}
+        if save_config is not None:
+            for fn in ("field-%d-0.out" % save_config, "moment-%d.out" % save_config):
+                files[fn] = open(os.path.join(td, fn)).read()
         odir = os.path.join(td, "output", "0")
         if os.path.isdir(odir):
             for fn in sorted(os.listdir(odir)):
@@ -495,3 +501,15 @@ def load_track_fixture(path):
         track.append((int(z["t_index"][i]), int(z["t_status"][i]), int(z["t_disp"][i]), hist))
     files = {str(n): str(z["file_" + str(n).replace(".", "_")]) for n in z["file_names"]}
     return m, z, track, files
+
+
+def config_text(phi_dense, pi_dense, fp32=False):
+    """SaveConfig's field file (fieldanalysis.cpp:252-275): one "Phi Pi" line
+    per interior site in pos() order, printf %.17e (%.9e for FP32 fields)."""
+    fmt = "%.9e %.9e\n" if fp32 else "%.17e %.17e\n"
+    return "".join(fmt % (float(a), float(b)) for a, b in zip(phi_dense, pi_dense))
+
+
+def moment_text(tau):
+    """SaveConfig's moment file: `fout << tau << endl` (six significant digits)."""
+    return "%g\n" % tau

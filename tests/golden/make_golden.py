@@ -83,6 +83,21 @@ def main_track():
         print("wrote", name, "tracked:", len(r["track"]), "files:", sorted(files))
 
 
+def main_config():
+    """SaveConfig (fieldanalysis.cpp:252-282) of the cfg0 fixture's final state."""
+    d = os.path.join(HERE, "config")
+    os.makedirs(d, exist_ok=True)
+    N = 8
+    m = O.model(N)
+    phi, pi = _fields(N, 1, 2.449489742783178, 0.4)
+    r = O.run_ref(m, phi, pi, 40, save_config=7)
+    np.savez_compressed(os.path.join(d, "saveconfig_n8_s40.npz"), N=N, steps=40, name=7,
+                        phi_out=r["phi"], pi_out=r["pi"],
+                        field_text=np.array(r["files"]["field-7-0.out"]),
+                        moment_text=np.array(r["files"]["moment-7.out"]))
+    print("wrote saveconfig_n8_s40")
+
+
 def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
@@ -103,3 +118,4 @@ def main():
 if __name__ == "__main__":
     main()
     main_track()
+    main_config()

@@ -91,3 +91,37 @@ def _cont(m, phi, pi, start, stop):
     d = np.ascontiguousarray(dtd[start:stop])
     O.lib().or_evolve(C.byref(p), O.dptr(rphi), O.dptr(rpi), stop - start, O.dptr(k), O.dptr(d))
     return O.from_ref(rphi, N), O.from_ref(rpi, N)
+
+
+def test_driver_checkpoint_round_trip(tmp_path):
+    # -saveconfig writes the reference's field-<n>-<ID>.out / moment-<n>.out
+    # (fieldanalysis.cpp:252-282) for the state the oracle reaches; a second
+    # run with -extractconfig resumes from it and saves the same state
+    N, seed, TMax = 16, 77, 1.0
+    m = O.model(N)
+    phi0 = float(np.sqrt(6.0))
+    base = [MAIN, "-runs", "1", "-Nt", str(N), "-at", "0.5", "-M", "1", "-Mmode", "1", "-L", "6",
+            "-Lmode", "-1", "-Kappa", "%.17g" % m["kappa"], "-T0", "0", "-TMax", "%g" % TMax,
+            "-dt", "0.1", "-TSO", "0.5", "-TMO", "1.0", "-Noise", "0.5", "-Ampl", "0",
+            "-QS", "%.17g" % (np.sqrt(3.0) / 0.5), "-Q2", "%.17g" % (np.sqrt(3.0) / 0.5),
+            "-phi0_0", "%.17g" % phi0, "-seed", str(seed), "-output", "0"]
+    r = subprocess.run(base + ["-saveconfig", "7"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, taus = O.time_factors(100)
+    steps = int(np.argmax(taus >= TMax)) + 1  # for (reset(); tau < TMax; evolve())
+    ophi, opi = O.initial_conditions(dict(m), seed, phi0)
+    ophi, opi = O.evolve(m, ophi, opi, steps)
+    field = (tmp_path / "field-7-0.out").read_text()
+    vals = np.array([float(t) for t in field.split()]).reshape(-1, 2)
+    # the format exactly (%.17e round-trips doubles); the state to 1e-10
+    # (the driver's IC goes through rocFFT, the oracle's through numpy)
+    assert field == O.config_text(vals[:, 0], vals[:, 1])
+    np.testing.assert_allclose(vals[:, 0], ophi, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(vals[:, 1], opi, rtol=1e-10, atol=1e-12)
+    assert (tmp_path / "moment-7.out").read_text() == O.moment_text(taus[steps - 1])
+    r = subprocess.run(base + ["-extractconfig", "7", "-saveconfig", "8"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "field-8-0.out").read_text() == field
+    assert (tmp_path / "moment-8.out").read_text() == (tmp_path / "moment-7.out").read_text()
