@@ -308,18 +308,22 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   const int N = c->p.N;
   const bool dp = c->p.precision == PN_FP64;
   if (N < 4) return false;
-  // measured on MI355X at 256^3 and 512^3 (ms per step, FP64: 0.072 / 0.542,
-  // FP32: 0.035 / 0.248); neighbouring shapes are within a few percent
+  // measured on MI355X at 256^3 and 512^3 (ms per step, FP64: 0.064 / 0.501,
+  // FP32: 0.034 / 0.230); neighbouring shapes are within a few percent
   if (dp) {
-    if (N <= 256 && N % 2 == 0) *s = {2, 6, 16};
+    if (N <= 256 && N % 2 == 0) *s = {2, 6, 32};
     else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64};
     else return false;
   } else {
-    if (N <= 512 && N % 4 == 0) *s = {4, 8, N <= 256 ? 16 : 64};
+    if (N <= 256 && N % 4 == 0) *s = {4, N >= 8 ? 10 : 8, 16};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 8, 16};
     else if (N <= 1024 && N % 8 == 0) *s = {8, 8, 64};
     else return false;
   }
   if (s->ZB > N) s->ZB = N;
+  // a tile's rows y0-1 .. y0+RT-2 must wrap into [0, N) with one fold
+  const int TY = s->RT - 2, YT = (N + TY - 1) / TY;
+  if ((YT - 1) * TY + s->RT - 2 >= 2 * N) return false;
   const size_t lds = (size_t)2 * s->RT * N * c->esize;
   return (N / s->W) * s->RT <= 1024 && lds <= 65536;
 }
@@ -348,7 +352,8 @@ static int launch_step2(pn_ctx *c, const Step2Shape &s, const double *dtk, const
   } else {
     CoefF32 k = coef32(c);
     const float a0 = (float)dtk[0], b0 = (float)dtd[0], a1 = (float)dtk[1], b1 = (float)dtd[1];
-    if (s.W == 4) launch_step2_t<float, CoefF32, true, 4, 8>(c, k, s.ZB, a0, b0, a1, b1);
+    if (s.W == 4 && s.RT == 10) launch_step2_t<float, CoefF32, true, 4, 10>(c, k, s.ZB, a0, b0, a1, b1);
+    else if (s.W == 4) launch_step2_t<float, CoefF32, true, 4, 8>(c, k, s.ZB, a0, b0, a1, b1);
     else launch_step2_t<float, CoefF32, true, 8, 8>(c, k, s.ZB, a0, b0, a1, b1);
   }
   PN_CHECK_LAUNCH(c);

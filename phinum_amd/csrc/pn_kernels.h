@@ -138,8 +138,9 @@ struct RedCoef {
 // steps.  Pi is read from `pi` and written to `pio`: a neighbouring tile's
 // stage A still needs the old Pi of the rows this tile writes.  Threads with
 // active == false only join the barriers (one per plane), which every thread
-// of the block must reach.
-template <typename T, typename C, bool POW2, int W>
+// of the block must reach.  PF issues the next plane's HBM reads before the
+// barrier (two more vector registers of W values per thread).
+template <typename T, typename C, bool POW2, int W, bool PF = true>
 __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *__restrict__ pi,
                                            T *__restrict__ pio, T *__restrict__ phin, T *lds, int N,
                                            int RT, int tx, int ty, bool active, int y0, int z0,
@@ -155,20 +156,30 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
   const long long P = (long long)N * N;
   const int row = y * N, rowp = yp * N, rowm = ym * N;
   auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
-  V an_m = {}, an_c = {};
+  // an_n, p_n: Phi of plane za+1 and Pi of plane za for the coming
+  // iteration, read from HBM before the previous iteration's barrier so that
+  // their latency overlaps the wait (the in-plane neighbours that follow are
+  // mostly L2 hits: the neighbouring rows' own reads)
+  V an_m = {}, an_c = {}, an_n = {}, p_n = {};
   if (active) {
     an_m = *(const V *)&phi[wz(z0 - 2) * P + row + x];
     an_c = *(const V *)&phi[wz(z0 - 1) * P + row + x];
+    if (PF) {
+      an_n = *(const V *)&phi[wz(z0) * P + row + x];
+      p_n = *(const V *)&pi[wz(z0 - 1) * P + row + x];
+    }
   }
   V b_m = an_m, b_c = an_c, pih = an_c;
   for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
-    V an_p = {}, p = {}, f1 = {};
+    V an_p = an_n, p = p_n, f1 = {};
     if (active) {
       const long long base = (long long)wz(za) * P;
-      an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
+      if (!PF) {
+        an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
+        p = *(const V *)&pi[base + row + x];
+      }
       const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
       const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
-      p = *(const V *)&pi[base + row + x];
       auto stage_a = [&](auto div) {
 #pragma unroll
         for (int j = 0; j < W; j++) {
@@ -227,6 +238,10 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
       const long long bbase = (long long)zb * P;
       __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
       __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
+    }
+    if (PF && active && za < z1) {
+      an_n = *(const V *)&phi[wz(za + 2) * P + row + x];
+      p_n = *(const V *)&pi[wz(za + 1) * P + row + x];
     }
     // The only barrier of the iteration.  LDS holds two planes: stage A of
     // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
