@@ -319,3 +319,47 @@ def test_islands_match_bfs(N, seed, mode):
                                    rtol=1e-12, atol=1e-14)
         members = np.nonzero(labels == r.index)[0]
         assert r.first_pos == members.min() + P
+
+
+def _check_centroids(rows, lab, N):
+    P = N * N
+    for r in rows:
+        sites = np.nonzero(lab == r.index)[0]
+        pk = r.peak_pos - P
+        px, py, pz = pk % N, (pk // N) % N, pk // P
+        x, y, z = sites % N, (sites // N) % N, sites // P
+        dx = (x - px + N // 2) % N - N // 2
+        dy = (y - py + N // 2) % N - N // 2
+        assert (r.nn, r.sum_dx, r.sum_dy, r.sum_dz) == (sites.size, dx.sum(), dy.sum(), (z - pz).sum())
+        assert r.cz == pz + (z - pz).sum() / sites.size
+        assert abs(r.cx - (px + dx.sum() / sites.size) % N) < 1e-12
+        assert abs(r.cy - (py + dy.sum() / sites.size) % N) < 1e-12
+
+
+def test_island_centroids_brute_force():
+    # the centroid is a new output (the reference keeps only the peak site):
+    # offsets from the peak, minimum image in x and y (periodic), plain in z
+    # (open), checked against a numpy restatement on a random lattice and on
+    # a planted island that straddles the periodic x and y edges
+    N = 16
+    m = O.model(N)
+    P = N * N
+    rng = np.random.default_rng(81)
+    phi = rng.standard_normal(N ** 3)
+    pi = 0.1 * rng.standard_normal(N ** 3)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        rows, lab, _ = L.islands(0, 1.5, labels=True)
+        assert len(rows) > 20
+        _check_centroids(rows, lab, N)
+        bg = np.clip(phi, -1.0, 1.0).reshape(N, N, N)  # background stays below thr = 3
+        for x in (14, 15, 0, 1):
+            for y in (15, 0):
+                bg[9, y, x] = 5.0
+        bg[9, 0, 15] = 6.0  # the peak
+        L.set_fields(bg.ravel(), pi)
+        rows, lab, _ = L.islands(0, 3.0, labels=True)
+        _check_centroids(rows, lab, N)
+    assert len(rows) == 1 and rows[0].nn == 8 and rows[0].peak_pos == P + 9 * P + 15
+    # x sites 14, 15, 0, 1 around 15: mean 15.5; y sites 15, 0 around 0: mean -0.5 -> 15.5
+    assert (rows[0].cx, rows[0].cy, rows[0].cz) == (15.5, 15.5, 9.0)
