@@ -35,12 +35,13 @@ struct CoefF32 {
   float inv_a2, c2, c4, c6;
 };
 
-// |f| in [2^-100, 2^100] or f == 0 puts q = f*f in [2^-200, 2^200] or +0,
-// which keeps q/6, q/kappa (kappa in [2^-100, 2^100]) and t*t/120 inside the
-// range where pn_div_fast equals the IEEE quotient.
+// |f| in [2^-100, 2^100) puts q = f*f in [2^-200, 2^200), which keeps q/6,
+// q/kappa (kappa in [2^-100, 2^100]) and t*t/120 inside the range where
+// pn_div_fast equals the IEEE quotient.  The test reads the exponent bits:
+// zero, subnormals, Inf and NaN fall outside and take the IEEE divisions.
 __device__ __forceinline__ bool fast_div_site(double f) {
-  const double a = __builtin_fabs(f);
-  return (a >= 0x1p-100 && a <= 0x1p+100) || a == 0.0;
+  const unsigned hi = (unsigned)((unsigned long long)__double_as_longlong(f) >> 32) & 0x7fffffffu;
+  return hi - 0x39B00000u < 0x0C800000u;
 }
 
 enum { DIV_IEEE = 0, DIV_FAST = 1, DIV_PER_SITE = 2 };
@@ -160,22 +161,26 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
   // iteration, read from HBM before the previous iteration's barrier so that
   // their latency overlaps the wait (the in-plane neighbours that follow are
   // mostly L2 hits: the neighbouring rows' own reads)
+  // plane offsets of za, za+1, za+2, advanced by one plane per iteration
+  const long long NP = (long long)N * P;
+  long long om = (long long)wz(z0 - 2) * P, o0 = (long long)wz(z0 - 1) * P,
+            o1 = (long long)wz(z0) * P, o2 = (long long)wz(z0 + 1) * P;
   V an_m = {}, an_c = {}, an_n = {}, p_n = {};
   if (active) {
-    an_m = *(const V *)&phi[wz(z0 - 2) * P + row + x];
-    an_c = *(const V *)&phi[wz(z0 - 1) * P + row + x];
+    an_m = *(const V *)&phi[om + row + x];
+    an_c = *(const V *)&phi[o0 + row + x];
     if (PF) {
-      an_n = *(const V *)&phi[wz(z0) * P + row + x];
-      p_n = *(const V *)&pi[wz(z0 - 1) * P + row + x];
+      an_n = *(const V *)&phi[o1 + row + x];
+      p_n = *(const V *)&pi[o0 + row + x];
     }
   }
   V b_m = an_m, b_c = an_c, pih = an_c;
   for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
     V an_p = an_n, p = p_n, f1 = {};
+    const long long base = o0;
     if (active) {
-      const long long base = (long long)wz(za) * P;
       if (!PF) {
-        an_p = *(const V *)&phi[wz(za + 1) * P + row + x];
+        an_p = *(const V *)&phi[o1 + row + x];
         p = *(const V *)&pi[base + row + x];
       }
       const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
@@ -235,14 +240,18 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
       } else {
         stage_b(std::integral_constant<int, DIV_IEEE>());
       }
-      const long long bbase = (long long)zb * P;
+      const long long bbase = om;  // plane zb = za - 1
       __builtin_nontemporal_store(q, (V *)&pio[bbase + row + x]);
       __builtin_nontemporal_store(f2, (V *)&phin[bbase + row + x]);
     }
     if (PF && active && za < z1) {
-      an_n = *(const V *)&phi[wz(za + 2) * P + row + x];
-      p_n = *(const V *)&pi[wz(za + 1) * P + row + x];
+      an_n = *(const V *)&phi[o2 + row + x];
+      p_n = *(const V *)&pi[o1 + row + x];
     }
+    om = o0;
+    o0 = o1;
+    o1 = o2;
+    o2 = o2 + P == NP ? 0 : o2 + P;
     // The only barrier of the iteration.  LDS holds two planes: stage A of
     // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
     // written one iteration earlier.  This barrier orders that write before
