@@ -120,8 +120,9 @@ def batch_leg(pn, steps, out_every=500):
     """64 realizations: 32 x 64^3, 24 x 128^3, 8 x 192^3, seeds 0..63, lambda = 1,
     g drawn uniformly from [0.1, 0.4] with numpy's default_rng(0); FP64, one
     launch per pair of steps for all of them (two-step tiles), 100-step HIP
-    graphs, tiles autotuned per size class.  Energies and the false-vacuum check every `out_every` steps
-    (outside the timing).  Returns a dict for the JSON line."""
+    graphs, tiles autotuned per size class.  Energies and the false-vacuum
+    check every `out_every` steps (outside the timing).  Returns a dict for the
+    JSON line."""
     sizes = [64] * 32 + [128] * 24 + [192] * 8
     g = np.random.default_rng(0).uniform(0.1, 0.4, len(sizes))
     reals = [dict(N=N, a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1,
@@ -224,11 +225,21 @@ def cpu_baseline(N, m, target_s=12.0):
     steps = max(3, int(target_s / max(probe / 3.0, 1e-6)))
     sec = lib.cb_run(N, threads, 2, steps, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
                      m["kappa"], 0.1, m["phi0"], C.byref(ck))
+    model_name = "unknown CPU"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model_name = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": threads,
             "kind": "port",
             "sample": "%d^3 FP64 leapfrog, %d timed steps (%.1f s) of the reference loop nest "
                       "(AoS + z halo, pos()/mod() per site, kick then drift, z slabs over %d threads), "
-                      "reference flags -O3 -ffast-math" % (N, steps, sec, threads)}
+                      "reference flags -O3 -ffast-math" % (N, steps, sec, threads),
+            "host": {"cpu": model_name, "nproc": os.cpu_count(),
+                     "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}}
 
 
 def main():
