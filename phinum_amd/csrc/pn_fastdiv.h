@@ -50,11 +50,67 @@ inline pn_recip pn_make_recip(double b, int *fast) {
   return r;
 }
 
-// RN(a/b) for |a| in [2^-600, 2^600] or a == +0, given pn_make_recip(b) with fast == 1
-PN_HD double pn_div_fast(double a, const pn_recip &r) {
-  const double q = __builtin_fma(a, r.zh, a * r.zl);
+// Markstein's correction (steps 2-3) of a faithful quotient q of a/b
+PN_HD double pn_div_correct(double a, const pn_recip &r, double q) {
   const double e = __builtin_fma(-r.b, q, a);
   return __builtin_fma(e, r.zh, q);
+}
+
+// RN(a/b) for |a| in [2^-600, 2^600] or a == +0, given pn_make_recip(b) with fast == 1
+PN_HD double pn_div_fast(double a, const pn_recip &r) {
+  return pn_div_correct(a, r, __builtin_fma(a, r.zh, a * r.zl));
+}
+
+// RN(a/b) in two operations, Brisebarre-Muller-Raina's step 1 alone, for
+// divisors that pass pn_bmr_exact (same range as pn_div_fast).  s = a*zh +
+// RN(a*zl) is within |a/b| 2^-105 (1 + 2^-54) of a/b, so RN(s) can differ from
+// RN(a/b) only where a/b lies that close to a rounding midpoint m.  With
+// a = A 2^ea, b = B 2^eb (A, B 53-bit integers) and m = (2K+1) 2^(E-53),
+// |a/b - m| = 2^(E-53) |N1| / B for the integer N1 = A 2^t - B (2K+1), t = 53
+// or 54; the bound then needs |N1| <= 4.  N1 is a multiple of 2^s (2^s the
+// largest power of two dividing B), so s >= 3 (6 = 3 2^1 and 120 = 15 2^3 give
+// s = 51 and 49) rules every a out; otherwise the few A with |N1| <= 4 are
+// solved for and tested.
+PN_HD double pn_div_bmr(double a, const pn_recip &r) { return __builtin_fma(a, r.zh, a * r.zl); }
+
+// host side: 1 when pn_div_bmr is correctly rounded for every dividend in the
+// fast range (requires pn_make_recip's fast == 1)
+inline int pn_bmr_exact(const pn_recip &r) {
+  int e;
+  const double fr = std::frexp(r.b, &e);
+  unsigned long long B = (unsigned long long)std::ldexp(fr, 53);
+  int s = 0;
+  while (!(B & 1ull)) {
+    B >>= 1;
+    s++;
+  }
+  if (s >= 3) return 1;
+  const unsigned long long Bo = B;  // odd part
+  for (int t = 53; t <= 54; t++) {
+    for (int c = -4; c <= 4; c++) {
+      if (c == 0 || c % (1 << s) != 0) continue;
+      const long long cc = c / (1 << s);
+      const int tt = t - s;  // A 2^tt - Bo M = cc with M odd
+      // inverse of 2^tt modulo Bo by exponentiation of the inverse of 2
+      const unsigned __int128 inv2 = (Bo + 1) / 2;
+      unsigned __int128 inv = 1;
+      for (int k = 0; k < tt; k++) inv = inv * inv2 % Bo;
+      const unsigned __int128 cm = (unsigned __int128)((cc % (long long)Bo + (long long)Bo) % (long long)Bo);
+      const unsigned long long A0 = (unsigned long long)(cm * inv % Bo);
+      const unsigned long long lo = 1ull << 52, hi = 1ull << 53;
+      unsigned long long A = A0;
+      if (A < lo) A += (lo - A + Bo - 1) / Bo * Bo;
+      for (; A < hi; A += Bo) {
+        const __int128 num = ((__int128)A << tt) - cc;
+        if (num % (__int128)Bo != 0 || ((num / (__int128)Bo) & 1) == 0) continue;
+        const double a = std::ldexp((double)A, -52);  // in [1, 2): a normal dividend
+        if (pn_div_bmr(a, r) != a / r.b) return 0;
+        if (pn_div_bmr(-a, r) != -a / r.b) return 0;
+        if (Bo == 1) break;
+      }
+    }
+  }
+  return 1;
 }
 
 // true when `a` lies where pn_div_fast is exact

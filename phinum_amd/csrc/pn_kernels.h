@@ -21,6 +21,7 @@ struct CoefF64 {
   double a2, inv_a2, c2, lmode, kappa;
   pn_recip r6, rk, r120;  // two-term reciprocals of 6, kappa and 120
   int fdiv;               // kappa lies where pn_div_fast is exact
+  int rk_bmr;             // q/kappa needs no correction step (pn_bmr_exact)
 };
 
 // host side: the reciprocals for the force divisions
@@ -29,7 +30,9 @@ inline void coef64_recips(CoefF64 &k) {
   k.r6 = pn_make_recip(6., &f6);
   k.rk = pn_make_recip(k.kappa, &fk);
   k.r120 = pn_make_recip(120., &f120);
-  k.fdiv = f6 && fk && f120;
+  // 6 and 120 always pass pn_bmr_exact (their significands carry 2^1 and 2^3)
+  k.fdiv = f6 && fk && f120 && pn_bmr_exact(k.r6) && pn_bmr_exact(k.r120);
+  k.rk_bmr = k.fdiv && pn_bmr_exact(k.rk);
 }
 struct CoefF32 {
   float inv_a2, c2, c4, c6;
@@ -64,9 +67,10 @@ __device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, dou
   double B, C;
   const bool fast = DIV == DIV_FAST || (DIV == DIV_PER_SITE && k.fdiv && fast_div_site(f));
   if (fast) {
-    B = k.lmode * pn_div_fast(q, k.r6);
-    const double t = pn_div_fast(q, k.rk);
-    C = pn_div_fast(t * t, k.r120);
+    B = k.lmode * pn_div_bmr(q, k.r6);
+    double t = pn_div_bmr(q, k.rk);
+    if (!k.rk_bmr) t = pn_div_correct(q, k.rk, t);
+    C = pn_div_bmr(t * t, k.r120);
   } else {
     B = k.lmode * (q / 6.);
     const double t = q / k.kappa;

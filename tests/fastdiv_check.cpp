@@ -3,7 +3,10 @@
 // quotient lies next to a rounding midpoint (the hard cases for correct
 // rounding), the range ends, zero, and the divisors of the leapfrog force
 // (6, 120, kappa) plus random ones.
-// usage: fastdiv_check <samples per divisor> <seed>; prints "mismatches N checked M"
+// pn_div_bmr (two operations) is checked the same way on every divisor that
+// pn_bmr_exact accepts, which must include 6 and 120.
+// usage: fastdiv_check <samples per divisor> <seed>; prints
+// "mismatches N checked M bmr_divisors D bmr_rejected R"
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -64,10 +67,15 @@ int main(int argc, char **argv) {
   divisors[nd++] = 0x1.fffffffffffffp+0;  // significand all ones
   divisors[nd++] = 0x1.0000000000001p+0;
   divisors[nd++] = 7.0;
+  // two divisors for which the two-operation form fails on some dividend:
+  // pn_bmr_exact must find that and reject them
+  const double bmr_fail[2] = {0x1.fff9af89f7dd3p+5, 0x1.defa6767e7f8bp-2};
+  divisors[nd++] = bmr_fail[0];
+  divisors[nd++] = bmr_fail[1];
   while (nd < 40) divisors[nd++] = rand_in(-99, 99);
   divisors[nd++] = 0x1p-100;
   divisors[nd++] = 0x1.8p+99;
-  long bad = 0, checked = 0;
+  long bad = 0, checked = 0, bmr_div = 0, bmr_rej = 0;
   {  // refused: negative divisor, -0 dividend
     int fast = 1;
     pn_make_recip(-7.0, &fast);
@@ -78,6 +86,17 @@ int main(int argc, char **argv) {
     const pn_recip r = pn_make_recip(divisors[d], &fast);
     if (!fast) continue;
     const double b = divisors[d];
+    const int bmr = pn_bmr_exact(r);
+    bmr_div += bmr;
+    bmr_rej += !bmr;
+    if ((b == 6.0 || b == 120.0) && !bmr) {
+      printf("pn_bmr_exact rejects %g\n", b);
+      bad++;
+    }
+    if ((b == bmr_fail[0] || b == bmr_fail[1]) && bmr) {
+      printf("pn_bmr_exact accepts %a\n", b);
+      bad++;
+    }
     auto check = [&](double a) {
       if (!pn_div_fast_ok(a)) return;
       const double want = a / b;
@@ -86,6 +105,14 @@ int main(int argc, char **argv) {
       if (bits(want) != bits(got)) {
         if (bad < 10) printf("mismatch a=%a b=%a want=%a got=%a\n", a, b, want, got);
         bad++;
+      }
+      if (bmr) {
+        const double g2 = pn_div_bmr(a, r);
+        checked++;
+        if (bits(want) != bits(g2)) {
+          if (bad < 10) printf("bmr mismatch a=%a b=%a want=%a got=%a\n", a, b, want, g2);
+          bad++;
+        }
       }
     };
     check(0.0);
@@ -143,6 +170,6 @@ int main(int argc, char **argv) {
       check(-k * b);
     }
   }
-  printf("mismatches %ld checked %ld\n", bad, checked);
+  printf("mismatches %ld checked %ld bmr_divisors %ld bmr_rejected %ld\n", bad, checked, bmr_div, bmr_rej);
   return bad != 0;
 }

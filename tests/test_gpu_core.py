@@ -363,3 +363,20 @@ def test_island_centroids_brute_force():
     assert len(rows) == 1 and rows[0].nn == 8 and rows[0].peak_pos == P + 9 * P + 15
     # x sites 14, 15, 0, 1 around 15: mean 15.5; y sites 15, 0 around 0: mean -0.5 -> 15.5
     assert (rows[0].cx, rows[0].cy, rows[0].cz) == (15.5, 15.5, 9.0)
+
+
+@pytest.mark.parametrize("kappa", [1.2247448713915889, 0.46775209018414249])
+@pytest.mark.parametrize("N", [16, 15, 64])
+def test_kappa_division_paths_bitwise(N, kappa):
+    # q/kappa takes the two-operation form when kappa passes pn_bmr_exact
+    # (sqrt(1.5) does) and the corrected four-operation form otherwise (the
+    # second value fails it); both must give the oracle's bits
+    m = dict(O.model(N), kappa=kappa)
+    phi, pi = _fields(N, 95 + N, phi0=0.8, amp=0.5)
+    ophi, opi = O.evolve(m, phi, pi, 6)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 6)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        L.evolve(dtk, dtd)
+        gphi, gpi = L.get_fields()
+    assert np.array_equal(_bits(gphi), _bits(ophi)) and np.array_equal(_bits(gpi), _bits(opi))
