@@ -299,7 +299,9 @@ static int launch_step(pn_ctx *c, int mode, double dtk, double dtd) {
 }
 
 // Two-step tile shape: W sites per thread, RT thread rows, ZB planes per block.
-// Full rows need N/W <= 128 threads per row; LDS holds 2*RT*N values (<= 64 KB).
+// Full rows need (N/W)*RT <= 1024 threads; LDS holds four planes of RT*N
+// values (two of stage A's results, two of Phi), up to the device's limit
+// (160 KB a block on MI355X).
 struct Step2Shape {
   int W, RT, ZB;
 };
@@ -308,15 +310,15 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   const int N = c->p.N;
   const bool dp = c->p.precision == PN_FP64;
   if (N < 4) return false;
-  // measured on MI355X at 256^3 and 512^3 (ms per step, FP64: 0.064 / 0.501,
-  // FP32: 0.034 / 0.230); neighbouring shapes are within a few percent
+  // measured on MI355X at 256^3 and 512^3 (ms per step, FP64: 0.059 / 0.477,
+  // FP32: 0.030 / 0.207); neighbouring shapes are within a few percent
   if (dp) {
     if (N <= 256 && N % 2 == 0) *s = {2, 6, 32};
     else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64};
     else return false;
   } else {
     if (N <= 256 && N % 4 == 0) *s = {4, N >= 8 ? 10 : 8, 16};
-    else if (N <= 512 && N % 4 == 0) *s = {4, 8, 16};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64};
     else if (N <= 1024 && N % 8 == 0) *s = {8, 8, 64};
     else return false;
   }
@@ -324,15 +326,24 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   // a tile's rows y0-1 .. y0+RT-2 must wrap into [0, N) with one fold
   const int TY = s->RT - 2, YT = (N + TY - 1) / TY;
   if ((YT - 1) * TY + s->RT - 2 >= 2 * N) return false;
-  const size_t lds = (size_t)2 * s->RT * N * c->esize;
-  return (N / s->W) * s->RT <= 1024 && lds <= 65536;
+  const size_t lds = (size_t)4 * s->RT * N * c->esize;
+  return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
 }
 
 template <typename T, typename C, bool POW2, int W, int RT>
 static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T dd1) {
   const int N = c->p.N;
   const int YT = (N + RT - 3) / (RT - 2), ZT = (N + ZB - 1) / ZB;
-  const size_t lds = (size_t)2 * RT * N * sizeof(T);
+  const size_t lds = (size_t)4 * RT * N * sizeof(T);
+  if (lds > 65536) {
+    // beyond 64 KB the launch needs the kernel's dynamic LDS limit raised (once)
+    static int raised = 0;
+    if (!raised) {
+      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+      raised = 1;
+    }
+  }
   hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
@@ -353,6 +364,7 @@ static int launch_step2(pn_ctx *c, const Step2Shape &s, const double *dtk, const
     CoefF32 k = coef32(c);
     const float a0 = (float)dtk[0], b0 = (float)dtd[0], a1 = (float)dtk[1], b1 = (float)dtd[1];
     if (s.W == 4 && s.RT == 10) launch_step2_t<float, CoefF32, true, 4, 10>(c, k, s.ZB, a0, b0, a1, b1);
+    else if (s.W == 4 && s.RT == 6) launch_step2_t<float, CoefF32, true, 4, 6>(c, k, s.ZB, a0, b0, a1, b1);
     else if (s.W == 4) launch_step2_t<float, CoefF32, true, 4, 8>(c, k, s.ZB, a0, b0, a1, b1);
     else launch_step2_t<float, CoefF32, true, 8, 8>(c, k, s.ZB, a0, b0, a1, b1);
   }
@@ -417,6 +429,8 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
     return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(pn_fail(nullptr, PN_ERR_HIP, "pn_create: stream creation failed"));
+  if (hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) != hipSuccess)
+    c->lds_max = 65536;
   size_t bytes = (size_t)c->n3 * c->esize;
   for (int i = 0; i < 2; i++)
     if (hipMalloc(&c->phi[i], bytes) != hipSuccess)

@@ -32,6 +32,7 @@ struct pn_ctx {
   void *pi;            // momenta, updated in place
   void *pi_alt;        // second Pi buffer for the two-step kernel (or NULL)
   int step2;           // pn_evolve advances pairs of steps in one pass
+  int lds_max;         // LDS bytes a block may use on this device
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch

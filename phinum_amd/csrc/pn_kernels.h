@@ -144,8 +144,11 @@ struct RedCoef {
 // stage A still needs the old Pi of the rows this tile writes.  Threads with
 // active == false only join the barriers (one per plane), which every thread
 // of the block must reach.  PF issues the next plane's HBM reads before the
-// barrier (two more vector registers of W values per thread).
-template <typename T, typename C, bool POW2, int W, bool PF = true>
+// barrier (two more vector registers of W values per thread).  LP keeps the
+// tile's Phi planes in LDS too (two more planes of RT*N values after the
+// intermediate ones): stage A then reads its x and y neighbours there instead
+// of from L2, except the one outside row of each edge row.
+template <typename T, typename C, bool POW2, int W, bool PF = true, bool LP = true>
 __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *__restrict__ pi,
                                            T *__restrict__ pio, T *__restrict__ phin, T *lds, int N,
                                            int RT, int tx, int ty, bool active, int y0, int z0,
@@ -178,6 +181,13 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
       p_n = *(const V *)&pi[o0 + row + x];
     }
   }
+  // Phi planes for LP: buffer (z - z0) & 1 holds plane z; the first plane is
+  // stored before the loop, each later one an iteration ahead of its use
+  T *lphi = lds + (size_t)2 * RT * N;
+  if (LP) {
+    if (active) *(V *)&lphi[(size_t)RT * N + ty * N + x] = an_c;
+    __syncthreads();
+  }
   V b_m = an_m, b_c = an_c, pih = an_c;
   for (int za = z0 - 1; za <= z1; za++) {  // stage A on plane za, stage B on za-1
     V an_p = an_n, p = p_n, f1 = {};
@@ -187,8 +197,21 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
         an_p = *(const V *)&phi[o1 + row + x];
         p = *(const V *)&pi[base + row + x];
       }
-      const T vxp = phi[base + row + xp], vxm = phi[base + row + xm];
-      const V vyp = *(const V *)&phi[base + rowp + x], vym = *(const V *)&phi[base + rowm + x];
+      T vxp, vxm;
+      V vyp, vym;
+      if (LP) {
+        *(V *)&lphi[(size_t)((za + 1 - z0) & 1) * RT * N + ty * N + x] = an_p;
+        const T *lp = lphi + (size_t)((za - z0) & 1) * RT * N;
+        vxp = lp[ty * N + xp];
+        vxm = lp[ty * N + xm];
+        vyp = ty + 1 < RT ? *(const V *)&lp[(ty + 1) * N + x] : *(const V *)&phi[base + rowp + x];
+        vym = ty > 0 ? *(const V *)&lp[(ty - 1) * N + x] : *(const V *)&phi[base + rowm + x];
+      } else {
+        vxp = phi[base + row + xp];
+        vxm = phi[base + row + xm];
+        vyp = *(const V *)&phi[base + rowp + x];
+        vym = *(const V *)&phi[base + rowm + x];
+      }
       auto stage_a = [&](auto div) {
 #pragma unroll
         for (int j = 0; j < W; j++) {
@@ -256,11 +279,13 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
     o0 = o1;
     o1 = o2;
     o2 = o2 + P == NP ? 0 : o2 + P;
-    // The only barrier of the iteration.  LDS holds two planes: stage A of
-    // plane za writes buffer (za & 1) while stage B reads buffer ((za-1) & 1),
-    // written one iteration earlier.  This barrier orders that write before
-    // the next iteration's stage B reads it, and this iteration's stage B
-    // reads before the next stage A overwrites the buffer they use.
+    // The only barrier of the iteration.  LDS holds two planes of stage A's
+    // results: stage A of plane za writes buffer (za & 1) while stage B reads
+    // buffer ((za-1) & 1), written one iteration earlier; with LP two planes
+    // of Phi likewise: plane za+1 is written while stage A reads plane za.
+    // This barrier orders those writes before the next iteration reads them,
+    // and this iteration's reads before the next one overwrites the buffers
+    // they used.
     __syncthreads();
     b_m = b_c;
     b_c = f1;

@@ -43,7 +43,7 @@ def test_fused_step_bitwise_fp64(N, steps, seed):
                                           (16, 11, pn.PN_FP64), (34, 6, pn.PN_FP64), (130, 3, pn.PN_FP64),
                                           (256, 4, pn.PN_FP64), (260, 2, pn.PN_FP64),
                                           (4, 3, pn.PN_FP32), (8, 9, pn.PN_FP32), (36, 5, pn.PN_FP32),
-                                          (256, 4, pn.PN_FP32), (520, 2, pn.PN_FP32)])
+                                          (256, 4, pn.PN_FP32), (260, 2, pn.PN_FP32), (520, 2, pn.PN_FP32)])
 def test_two_step_kernel_bitwise_equals_single_steps(N, steps, prec):
     # temporal blocking changes only the evaluation order across sites
     m = O.model(N)
