@@ -240,13 +240,13 @@ __global__ __launch_bounds__(kB2, 6) void k_batch_step2(const T *__restrict__ ph
     const T dk0 = (T)dtk_tab[tb], dd0 = (T)dtd_tab[tb], dk1 = (T)dtk_tab[tb + 1], dd1 = (T)dtd_tab[tb + 1];
     if constexpr (sizeof(T) == 8) {
       if (d.pow2)
-        step2_tile<T, CoefF64, true, W, false, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+        step2_tile<T, CoefF64, true, W, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
                                         z0, z1, d.k64, dk0, dd0, dk1, dd1);
       else
-        step2_tile<T, CoefF64, false, W, false, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active,
+        step2_tile<T, CoefF64, false, W, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active,
                                          y0, z0, z1, d.k64, dk0, dd0, dk1, dd1);
     } else {
-      step2_tile<T, CoefF32, true, W, false, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+      step2_tile<T, CoefF32, true, W, false>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
                                       z0, z1, d.k32, dk0, dd0, dk1, dd1);
     }
   }
@@ -273,10 +273,8 @@ static int step2_rows(const pn_batch *b, int N) {
   return rt >= 3 ? rt : 0;
 }
 
-// two planes of stage A results in LDS: the tiles read their Phi neighbours
-// from L2 (the LDS-plane form of step2_tile does not compile for this kernel
-// with the ROCm 7.2 compiler: it emits an illegal v_cmp on src_shared_base)
-static size_t step2_lds(const pn_batch *b) { return (size_t)2 * kB2 * vec_width(b) * b->esize; }
+// four planes of the tile in LDS: two of stage A results, two of Phi
+static size_t step2_lds(const pn_batch *b) { return (size_t)4 * kB2 * vec_width(b) * b->esize; }
 
 // rebuild descriptors and the tile list; mask (optional) selects the
 // realizations that get tiles

@@ -150,10 +150,14 @@ struct RedCoef {
 // of from L2, except the one outside row of each edge row.
 template <typename T, typename C, bool POW2, int W, bool PF = true, bool LP = true>
 __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *__restrict__ pi,
-                                           T *__restrict__ pio, T *__restrict__ phin, T *lds, int N,
+                                           T *__restrict__ pio, T *__restrict__ phin, T *lds_g, int N,
                                            int RT, int tx, int ty, bool active, int y0, int z0,
                                            int z1, const C &k, T dk0, T dd0, T dk1, T dd1) {
   typedef typename VecT<T, W>::type V;
+  // LDS through address-space-3 pointers: ds_read/ds_write without a flat path
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(3))) V LV;
+  LT *lds = (LT *)lds_g;
   const int TY = RT - 2;
   const int x = W * tx;
   int y = y0 - 1 + ty;
@@ -183,9 +187,9 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
   }
   // Phi planes for LP: buffer (z - z0) & 1 holds plane z; the first plane is
   // stored before the loop, each later one an iteration ahead of its use
-  T *lphi = lds + (size_t)2 * RT * N;
+  LT *lphi = lds + (size_t)2 * RT * N;
   if (LP) {
-    if (active) *(V *)&lphi[(size_t)RT * N + ty * N + x] = an_c;
+    if (active) *(LV *)&lphi[(size_t)RT * N + ty * N + x] = an_c;
     __syncthreads();
   }
   V b_m = an_m, b_c = an_c, pih = an_c;
@@ -200,12 +204,12 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
       T vxp, vxm;
       V vyp, vym;
       if (LP) {
-        *(V *)&lphi[(size_t)((za + 1 - z0) & 1) * RT * N + ty * N + x] = an_p;
-        const T *lp = lphi + (size_t)((za - z0) & 1) * RT * N;
+        *(LV *)&lphi[(size_t)((za + 1 - z0) & 1) * RT * N + ty * N + x] = an_p;
+        const LT *lp = lphi + (size_t)((za - z0) & 1) * RT * N;
         vxp = lp[ty * N + xp];
         vxm = lp[ty * N + xm];
-        vyp = ty + 1 < RT ? *(const V *)&lp[(ty + 1) * N + x] : *(const V *)&phi[base + rowp + x];
-        vym = ty > 0 ? *(const V *)&lp[(ty - 1) * N + x] : *(const V *)&phi[base + rowm + x];
+        vyp = ty + 1 < RT ? *(const LV *)&lp[(ty + 1) * N + x] : *(const V *)&phi[base + rowp + x];
+        vym = ty > 0 ? *(const LV *)&lp[(ty - 1) * N + x] : *(const V *)&phi[base + rowm + x];
       } else {
         vxp = phi[base + row + xp];
         vxm = phi[base + row + xm];
@@ -235,14 +239,14 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
       } else {
         stage_a(std::integral_constant<int, DIV_IEEE>());
       }
-      T *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
-      *(V *)&buf[ty * N + x] = f1;
+      LT *buf = lds + (size_t)((za - z0 + 2) & 1) * RT * N;
+      *(LV *)&buf[ty * N + x] = f1;
     }
     if (za >= z0 + 1 && inner) {
       const int zb = za - 1;
-      const T *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
+      const LT *bb = lds + (size_t)((zb - z0 + 2) & 1) * RT * N;
       const T bxp = bb[ty * N + xp], bxm = bb[ty * N + xm];
-      const V byp = *(const V *)&bb[(ty + 1) * N + x], bym = *(const V *)&bb[(ty - 1) * N + x];
+      const V byp = *(const LV *)&bb[(ty + 1) * N + x], bym = *(const LV *)&bb[(ty - 1) * N + x];
       V q = pih, f2;
       auto stage_b = [&](auto div) {
 #pragma unroll
