@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 METRIC = "site-updates/s & % HBM roofline, FP32-comp vs FP64 at dE/E<=1e-6"
 
 
@@ -73,15 +74,22 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True):
     return total, np.array(energies), on
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE scaled by a plain copy) and written into profiles/<name>.md as
-    `traffic_bytes_per_launch = <int>`; None when that summary is absent."""
+def profiled_value(name, key):
+    """A per-launch figure measured with rocprofv3 --pmc and written into
+    profiles/<name>.md as `<key> = <number>`: `traffic_bytes_per_launch`
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), `fp64_flop_per_launch`
+    (64 x (2 FMA + ADD + MUL + TRANS) FP64 wave instructions) or
+    `valu_busy_percent`.  None when the summary or the line is absent."""
     path = os.path.join(ROOT, "profiles", name + ".md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
-    return int(m.group(1)) if m else None
+    m = re.search(r"%s = ([0-9.]+)" % key, open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def profiled_traffic(name):
+    t = profiled_value(name, "traffic_bytes_per_launch")
+    return int(t) if t else None
 
 
 def roofline(kernel, N, ms_total, steps, chunk, two_step, esize, profile=None):
@@ -113,6 +121,15 @@ def roofline(kernel, N, ms_total, steps, chunk, two_step, esize, profile=None):
             out["traffic"] = t
             out["traffic_over_algorithmic"] = t / algo
             out["traffic_source"] = "profiles/%s.md" % profile
+        flop = profiled_value(profile, "fp64_flop_per_launch")
+        if flop and esize == 8:
+            # FP64 FLOP counted by rocprofv3 over the live launch time
+            tf = flop / (per_launch_ms * 1e-3) / 1e12
+            out["fp64_valu"] = {"flop_per_launch": int(flop), "achieved": tf,
+                                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": tf / FP64_PEAK_TFLOPS,
+                                "valu_busy_percent": profiled_value(profile, "valu_busy_percent"),
+                                "source": "profiles/%s.md" % profile}
     return out
 
 
