@@ -36,10 +36,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def model(lam=1.0, g=0.2):
+def model(lam=1.0, g=0.2, phi0=1.0):
+    """The potential V = phi^2/2 - lam phi^4/4 + g phi^6/6 in the reference's
+    units (Lambda = 6 lam, Kappa^2 = Lambda^2 / (120 g), phi_ref = sqrt(Lambda) phi)."""
     L = 6.0 * lam
     return dict(a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1, kappa=float(np.sqrt(L * L / (120.0 * g))),
-                phi0=float(np.sqrt(L)) * 1.0, Lambda=L)
+                phi0=float(np.sqrt(L)) * phi0, Lambda=L)
 
 
 def launches(steps, chunk, two_step):
@@ -131,6 +133,84 @@ def roofline(kernel, N, ms_total, steps, chunk, two_step, esize, profile=None):
                                 "valu_busy_percent": profiled_value(profile, "valu_busy_percent"),
                                 "source": "profiles/%s.md" % profile}
     return out
+
+
+def precision_leg(pn, N, steps, out_every, seed=1234):
+    """Precision form: N^3, lambda = 1.2, g = 0.25, phi0 = 1.2 plus seeded
+    vacuum fluctuations, `steps` steps.  Phi and Pi stored and stepped in FP32
+    (energy and spectrum sums in FP64, compensated) against the same run in
+    FP64 from the same initial condition.  Per output: relative energy
+    difference, max |dphi| (absolute and over max |phi|), relative L2 field
+    errors, and the spectrum's relative error (max over the shells below the
+    initial cutoff QS, where the initial condition put power, of
+    |F32 - F64| / F64).  A third run, FP64 from the FP64 initial condition with
+    every phi moved up by one ulp, measures how fast this chaotic system
+    separates trajectories that start 1e-16 apart."""
+    m = model(lam=1.2, g=0.25, phi0=1.2)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    ic = pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"])
+    rows = []
+    ms = {"fp64": 0.0, "fp32": 0.0}
+
+    def lattice(prec):
+        return pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"], precision=prec)
+
+    with lattice(pn.PN_FP64) as L64, lattice(pn.PN_FP32) as L32, lattice(pn.PN_FP64) as Lc:
+        t0 = time.time()
+        L64.initial_conditions(ic)
+        L32.initial_conditions(ic)
+        log("precision %d^3: initial conditions in %.1fs" % (N, time.time() - t0))
+        phi, pi = L64.get_fields()
+        Lc.set_fields(np.nextafter(phi, np.inf), pi)
+        del phi, pi
+
+        def record(step):
+            e64, e32, ec = (L.volume_averages().energy for L in (L64, L32, Lc))
+            d = L32.field_diff(L64)
+            dc = Lc.field_diff(L64)
+            s64, s32 = L64.spectrum(), L32.spectrum()
+            p = np.arange(s64["nbins"]) * s64["binsize"]
+            sel = (s64["counts"] > 0) & (p < ic.QS)
+            spec = float(np.max(np.abs(s32["F"][sel] - s64["F"][sel]) / s64["F"][sel]))
+            rows.append(dict(step=step, e64=e64, e32=e32, dE=abs(e32 - e64) / abs(e64),
+                             max_dphi=d["max_dphi"], max_dphi_rel=d["max_dphi"] / d["max_phi"],
+                             l2_dphi=d["l2_dphi"], l2_dpi=d["l2_dpi"], spec=spec,
+                             ulp_dE=abs(ec - e64) / abs(e64), ulp_l2_dphi=dc["l2_dphi"],
+                             ulp_max_dphi=dc["max_dphi"]))
+            log("precision %d^3 step %d: fp32 dE/E %.3g, max|dphi| %.3g, L2 %.3g, spectrum %.3g; "
+                "fp64+1ulp dE/E %.3g, L2 %.3g" % (N, step, rows[-1]["dE"], d["max_dphi"], d["l2_dphi"], spec,
+                                                   rows[-1]["ulp_dE"], dc["l2_dphi"]))
+
+        record(0)
+        s = 0
+        while s < steps:
+            n = min(out_every, steps - s)
+            ms["fp64"] += L64.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+            ms["fp32"] += L32.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+            Lc.evolve(dtk[s:s + n], dtd[s:s + n])
+            s += n
+            record(s)
+        on64, on32 = L64.two_step(), L32.two_step()
+    dE = np.array([r["dE"] for r in rows])
+    over = np.nonzero(dE > 1e-6)[0]
+    e64 = np.array([r["e64"] for r in rows])
+    e32 = np.array([r["e32"] for r in rows])
+    v64 = float(N) ** 3 * steps / (ms["fp64"] * 1e-3)
+    v32 = float(N) ** 3 * steps / (ms["fp32"] * 1e-3)
+    return {"N": N, "steps": steps, "out_every": out_every, "lambda": 1.2, "g": 0.25, "phi0": 1.2,
+            "seed": seed, "fp64_two_step_kernel": on64, "fp32_two_step_kernel": on32,
+            "fp64_ms_per_step": ms["fp64"] / steps, "fp32_ms_per_step": ms["fp32"] / steps,
+            "fp64_site_updates_per_s": v64, "fp32_site_updates_per_s": v32, "ratio_fp32_vs_fp64": v32 / v64,
+            "steps_with_dE_over_E_le_1e-6": int(steps if over.size == 0 else rows[over[0] - 1]["step"]),
+            "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
+            "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
+            "per_output": {k: [float("%.3g" % r[k]) for r in rows]
+                           for k in ("step", "dE", "max_dphi", "max_dphi_rel", "l2_dphi", "l2_dpi", "spec",
+                                     "ulp_dE", "ulp_l2_dphi", "ulp_max_dphi")},
+            "per_output_keys": "dE = |E32 - E64| / E64; max_dphi = max |phi32 - phi64|; max_dphi_rel = "
+                               "max_dphi / max |phi64|; l2_dphi, l2_dpi = relative L2 field errors; spec = "
+                               "max over shells below QS of |F32 - F64| / F64; ulp_* = the same for an FP64 "
+                               "run started with every phi one ulp higher, against the FP64 run"}
 
 
 def batch_leg(pn, steps, out_every=500):
@@ -267,6 +347,11 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--out-every", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--precision-size", type=int, default=1024)
+    ap.add_argument("--precision-steps", type=int, default=10000)
+    ap.add_argument("--precision-every", type=int, default=1000)
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only the precision form (FP32 against FP64 at --precision-size)")
     ap.add_argument("--single-step", action="store_true", help="time the one-step kernel only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
@@ -291,6 +376,10 @@ def main():
 
     import phinum_amd as pn
 
+    if args.precision_only:
+        print(json.dumps(precision_leg(pn, args.precision_size, args.precision_steps,
+                                       args.precision_every)), flush=True)
+        return
     N, K, W = args.size, args.steps, args.warmup
     two = not args.single_step
     m = model()
@@ -348,6 +437,9 @@ def main():
                             "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
                             "roofline": roofline("k_step2<double, CoefF64, true, 4, 6>", N5, ms5, 200,
                                                  200, on5, 8, "k_step2_fp64_512cube")}
+        log("precision form %d^3" % args.precision_size)
+        line["precision_%d" % args.precision_size] = precision_leg(
+            pn, args.precision_size, args.precision_steps, args.precision_every)
         log("batch of 64 realizations")
         line["batch_64"] = batch_leg(pn, args.batch_steps)
         log("adjoint 128^3")

@@ -155,6 +155,12 @@ int pn_ic_modes(int N, double a, const pn_ic_params *ic, double *A, double *E);
  * host, rocFFT c2r on the device, /Norm (clusterfft.cpp:150) times NormFactor
  * (initialconditions.cpp:643) plus the background, into Phi and Pi. */
 int pn_initial_conditions(pn_ctx *c, const pn_ic_params *ic);
+/* The modes of pn_ic_modes for the context's N and a, drawn on its device and
+ * copied into the caller's A and E (no reference counterpart: the same
+ * drand48 stream, polar Box-Muller draws and ensureRealResult completion,
+ * found in parallel; the draws and mode positions are exact, log, sincos and
+ * hypot are the device's and may differ from the host's in the last bit). */
+int pn_ic_modes_device(pn_ctx *c, const pn_ic_params *ic, double *A, double *E);
 
 /* ---- dynamics (dynamics.cpp) ------------------------------------------ */
 
@@ -174,6 +180,10 @@ int pn_sync(pn_ctx *c);
  * of steps in one pass over HBM (temporal blocking); 0 makes every step its
  * own launch.  Both give bit-identical fields. */
 #define PN_OPT_TWO_STEP 1
+/* PN_OPT_IC_ON_DEVICE: 1 (default) makes pn_initial_conditions draw the modes
+ * on the device (pn_ic_modes_device); 0 draws them on the host (pn_ic_modes)
+ * and uploads them. */
+#define PN_OPT_IC_ON_DEVICE 2
 int pn_set_option(pn_ctx *c, int option, int value);
 int pn_get_option(pn_ctx *c, int option, int *value);
 /* pn_evolve bracketed by HIP events on the context's stream; waits for the
@@ -193,6 +203,16 @@ int pn_spectrum_nbins(int N, double a);
  * fftXtoP_direct (clusterfft.cpp:185-225): rocFFT r2c of Phi and Pi, times
  * a^3, binned by a hand-written kernel. */
 int pn_spectrum_get(pn_ctx *c, pn_spectrum *out);
+/* Field difference of lattice a against lattice b (same N and device, either
+ * precision), both converted to double: max |phi_a - phi_b|, max |pi_a - pi_b|,
+ * max |phi_b|, max |pi_b| and the compensated sums of (phi_a - phi_b)^2,
+ * phi_b^2, (pi_a - pi_b)^2, pi_b^2.  The precision form's field error against
+ * an FP64 run; no reference counterpart. */
+typedef struct pn_field_diff {
+  double max_dphi, max_dpi, max_phi, max_pi;
+  double sum_dphi2, sum_phi2, sum_dpi2, sum_pi2;
+} pn_field_diff;
+int pn_field_diff_get(pn_ctx *a, pn_ctx *b, pn_field_diff *out);
 /* fftXtoP_direct (clusterfft.cpp:185-225): the a^3-scaled r2c transform of
  * Phi (field 0) or Pi (field 1) as interleaved complex values in the
  * reference's p-space order py + px*(N/2+1) + pz*N*(N/2+1), N*N*(N/2+1)

@@ -23,6 +23,7 @@ PN_ERR_CAPACITY = -6
 PN_FP64, PN_FP32 = 0, 1
 PN_LAYOUT_REF, PN_LAYOUT_DENSE = 0, 1
 PN_OPT_TWO_STEP = 1
+PN_OPT_IC_ON_DEVICE = 2
 
 
 class PnParams(C.Structure):
@@ -86,6 +87,11 @@ class PnCatalog(C.Structure):
                 ("islands", C.POINTER(PnIsland)), ("labels", C.POINTER(C.c_int))]
 
 
+class PnFieldDiff(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("max_dphi", "max_dpi", "max_phi", "max_pi",
+                                          "sum_dphi2", "sum_phi2", "sum_dpi2", "sum_pi2")]
+
+
 class PnTrackEntry(C.Structure):
     _fields_ = [("at_time", C.c_double), ("center_pos", C.c_longlong), ("peak_value", C.c_double),
                 ("avg_phi", C.c_double), ("volume", C.c_double), ("pot_en", C.c_double),
@@ -138,8 +144,10 @@ def lib():
         "pn_tracker_count": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "pn_tracker_get": (C.c_int, [C.c_void_p, C.c_int, P(PnTrackedIsland)]),
         "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
+        "pn_field_diff_get": (C.c_int, [C.c_void_p, C.c_void_p, P(PnFieldDiff)]),
         "pn_ic_modes": (C.c_int, [C.c_int, C.c_double, P(PnIcParams), dp, dp]),
         "pn_initial_conditions": (C.c_int, [C.c_void_p, P(PnIcParams)]),
+        "pn_ic_modes_device": (C.c_int, [C.c_void_p, P(PnIcParams), dp, dp]),
         "pn_set_couplings": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double]),
         "pn_batch_create": (C.c_int, [C.c_int, P(PnRealization), C.c_int, C.c_int, P(C.c_void_p)]),
         "pn_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -326,6 +334,34 @@ class Lattice:
     def initial_conditions(self, ic):
         """SetInitialConditions (initialize==3) with a PnIcParams."""
         self._chk(self.L.pn_initial_conditions(self.h, C.byref(ic)))
+
+    def ic_on_device(self, on=None):
+        """Gets or sets PN_OPT_IC_ON_DEVICE (modes drawn on the device, the
+        default, or on the host)."""
+        if on is not None:
+            self._chk(self.L.pn_set_option(self.h, PN_OPT_IC_ON_DEVICE, int(bool(on))))
+        v = C.c_int()
+        self._chk(self.L.pn_get_option(self.h, PN_OPT_IC_ON_DEVICE, C.byref(v)))
+        return bool(v.value)
+
+    def ic_modes_device(self, ic):
+        """(A, E) modes of ic_modes drawn on this lattice's device."""
+        N = self.N
+        H = N // 2 + 1
+        A = np.empty(2 * N * N * H)
+        E = np.empty(2 * N * N * H)
+        self._chk(self.L.pn_ic_modes_device(self.h, C.byref(ic), _dptr(A), _dptr(E)))
+        return A.view(np.complex128).reshape(N, N, H), E.view(np.complex128).reshape(N, N, H)
+
+    def field_diff(self, other):
+        """Field error of this lattice against `other` (same N, either
+        precision): maxima, and relative L2 errors sqrt(sum d^2 / sum b^2)."""
+        d = PnFieldDiff()
+        self._chk(self.L.pn_field_diff_get(self.h, other.h, C.byref(d)))
+        out = {n: getattr(d, n) for n, _ in PnFieldDiff._fields_}
+        out["l2_dphi"] = float(np.sqrt(d.sum_dphi2 / d.sum_phi2)) if d.sum_phi2 > 0 else float(d.sum_dphi2 > 0)
+        out["l2_dpi"] = float(np.sqrt(d.sum_dpi2 / d.sum_pi2)) if d.sum_pi2 > 0 else float(d.sum_dpi2 > 0)
+        return out
 
     def energy_density(self):
         rho = np.empty(self.N ** 3)

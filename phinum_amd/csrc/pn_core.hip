@@ -200,8 +200,8 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
 // Two leapfrog steps per pass over HBM (temporal blocking): one block per
 // tile of RT full x rows (N/W threads of W sites each) and ZB planes; the
 // tile's work is step2_tile.  Needs N % W == 0, (N/W)*RT <= 1024 and
-// 2*RT*N values of LDS.
-template <typename T, typename C, bool POW2, int W, int RT>
+// 4*RT*N values of LDS (2*RT*N without the Phi planes, LP false).
+template <typename T, typename C, bool POW2, int W, int RT, bool LP = true>
 __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const T *__restrict__ pi,
                                                 T *__restrict__ pio, T *__restrict__ phin, int N,
                                                 int ZB, C k, T dk0, T dd0, T dk1, T dd1) {
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int yt = tile % YT, zt = tile / YT;
   const int z0 = zt * ZB;
-  step2_tile<T, C, POW2, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+  step2_tile<T, C, POW2, W, true, LP>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
                             (int)threadIdx.y, true, yt * (RT - 2), z0, min(z0 + ZB, N), k, dk0, dd0,
                             dk1, dd1);
 }
@@ -300,10 +300,10 @@ static int launch_step(pn_ctx *c, int mode, double dtk, double dtd) {
 
 // Two-step tile shape: W sites per thread, RT thread rows, ZB planes per block.
 // Full rows need (N/W)*RT <= 1024 threads; LDS holds four planes of RT*N
-// values (two of stage A's results, two of Phi), up to the device's limit
-// (160 KB a block on MI355X).
+// values (two of stage A's results, two of Phi; LP 0: the first two only), up
+// to the device's limit (160 KB a block on MI355X).
 struct Step2Shape {
-  int W, RT, ZB;
+  int W, RT, ZB, LP;
 };
 
 static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
@@ -313,38 +313,41 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   // measured on MI355X at 256^3 and 512^3 (ms per step, FP64: 0.059 / 0.477,
   // FP32: 0.030 / 0.207); neighbouring shapes are within a few percent
   if (dp) {
-    if (N <= 256 && N % 2 == 0) *s = {2, 6, 32};
-    else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64};
+    if (N <= 256 && N % 2 == 0) *s = {2, 6, 32, 1};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64, 1};
+    // at N = 1024, 256 threads a row leave room for four rows only (two output
+    // rows per tile: stage A runs on twice the output rows)
+    else if (N <= 1024 && N % 4 == 0) *s = {4, 4, 64, 1};
     else return false;
   } else {
-    if (N <= 256 && N % 4 == 0) *s = {4, N >= 8 ? 10 : 8, 16};
-    else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64};
-    else if (N <= 1024 && N % 8 == 0) *s = {8, 8, 64};
+    if (N <= 256 && N % 4 == 0) *s = {4, N >= 8 ? 10 : 8, 16, 1};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 6, 64, 1};
+    else if (N <= 1024 && N % 8 == 0) *s = {8, 8, 64, 1};
     else return false;
   }
   if (s->ZB > N) s->ZB = N;
   // a tile's rows y0-1 .. y0+RT-2 must wrap into [0, N) with one fold
   const int TY = s->RT - 2, YT = (N + TY - 1) / TY;
   if ((YT - 1) * TY + s->RT - 2 >= 2 * N) return false;
-  const size_t lds = (size_t)4 * s->RT * N * c->esize;
+  const size_t lds = (size_t)(s->LP ? 4 : 2) * s->RT * N * c->esize;
   return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
 }
 
-template <typename T, typename C, bool POW2, int W, int RT>
+template <typename T, typename C, bool POW2, int W, int RT, bool LP = true>
 static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T dd1) {
   const int N = c->p.N;
   const int YT = (N + RT - 3) / (RT - 2), ZT = (N + ZB - 1) / ZB;
-  const size_t lds = (size_t)4 * RT * N * sizeof(T);
+  const size_t lds = (size_t)(LP ? 4 : 2) * RT * N * sizeof(T);
   if (lds > 65536) {
     // beyond 64 KB the launch needs the kernel's dynamic LDS limit raised (once)
     static int raised = 0;
     if (!raised) {
-      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT>,
+      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT, LP>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
       raised = 1;
     }
   }
-  hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
+  hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT, LP>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
 }
@@ -356,9 +359,12 @@ static int launch_step2(pn_ctx *c, const Step2Shape &s, const double *dtk, const
     if (s.W == 2) {
       if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
       else launch_step2_t<double, CoefF64, false, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-    } else {
+    } else if (s.RT == 6) {
       if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
       else launch_step2_t<double, CoefF64, false, 4, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+    } else {
+      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 4>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
+      else launch_step2_t<double, CoefF64, false, 4, 4>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
     }
   } else {
     CoefF32 k = coef32(c);
@@ -644,6 +650,10 @@ extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
     c->step2 = value ? 1 : 0;
     return PN_OK;
   }
+  if (option == PN_OPT_IC_ON_DEVICE) {
+    c->ic_device = value ? 1 : 0;
+    return PN_OK;
+  }
   return pn_fail(c, PN_ERR_ARG, "pn_set_option: unknown option %d", option);
 }
 
@@ -651,6 +661,10 @@ extern "C" int pn_get_option(pn_ctx *c, int option, int *value) {
   if (!c || !value) return pn_fail(c, PN_ERR_ARG, "pn_get_option: NULL argument");
   if (option == PN_OPT_TWO_STEP) {
     *value = c->step2;
+    return PN_OK;
+  }
+  if (option == PN_OPT_IC_ON_DEVICE) {
+    *value = c->ic_device;
     return PN_OK;
   }
   return pn_fail(c, PN_ERR_ARG, "pn_get_option: unknown option %d", option);
@@ -865,7 +879,7 @@ extern "C" int pn_volume_averages(pn_ctx *c, pn_averages *out) {
   int rc = run_volume(c);
   if (rc != PN_OK) return rc;
   const double *s = c->host_pin;
-  const int vol = c->p.N * c->p.N * c->p.N;
+  const long long vol = (long long)c->p.N * c->p.N * c->p.N;  // N^3 overflows int above 1290
   // final arithmetic as fieldanalysis.cpp:619-638 (one rank: /nodes == /1)
   double phi1 = s[0], pi1 = s[1], phi2 = s[2], pi2 = s[3], phipi = s[4], phi4 = s[5], phi6 = s[6];
   double G = s[7], I = s[8];
@@ -930,5 +944,119 @@ extern "C" int pn_energy_density(pn_ctx *c, double *rho) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_energy_density: %s", hipGetErrorString(e));
+  return PN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field difference of two lattices: the precision form's field error against
+// an FP64 run (no reference counterpart)
+// ---------------------------------------------------------------------------
+constexpr int kDiffVals = 12;  // 4 maxima, 4 compensated sums (hi, lo)
+
+__device__ __forceinline__ double nan_max(double m, double v) { return (v > m || v != v) ? v : m; }
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_field_diff(const TA *__restrict__ pa, const TA *__restrict__ qa,
+                                                    const TB *__restrict__ pb, const TB *__restrict__ qb,
+                                                    long long n3, double *__restrict__ part) {
+  // mx: max|dphi|, max|dpi|, max|phi_b|, max|pi_b|;
+  // sums: (dphi)^2, phi_b^2, (dpi)^2, pi_b^2
+  double mx[4] = {0.0, 0.0, 0.0, 0.0}, hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+    const double fa = (double)pa[i], fb = (double)pb[i], ga = (double)qa[i], gb = (double)qb[i];
+    const double df = fa - fb, dg = ga - gb;
+    mx[0] = nan_max(mx[0], fabs(df));
+    mx[1] = nan_max(mx[1], fabs(dg));
+    mx[2] = nan_max(mx[2], fabs(fb));
+    mx[3] = nan_max(mx[3], fabs(gb));
+    two_acc(hi[0], lo[0], df * df);
+    two_acc(hi[1], lo[1], fb * fb);
+    two_acc(hi[2], lo[2], dg * dg);
+    two_acc(hi[3], lo[3], gb * gb);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      mx[s] = nan_max(mx[s], __shfl_down(mx[s], off, 64));
+      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
+      two_merge(hi[s], lo[s], h2, l2);
+    }
+  }
+  __shared__ double sh[4][kDiffVals];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int s = 0; s < 4; s++) {
+      sh[w][s] = mx[s];
+      sh[w][4 + 2 * s] = hi[s];
+      sh[w][5 + 2 * s] = lo[s];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ww = 1; ww < (int)(blockDim.x >> 6); ww++)
+      for (int s = 0; s < 4; s++) {
+        sh[0][s] = nan_max(sh[0][s], sh[ww][s]);
+        two_merge(sh[0][4 + 2 * s], sh[0][5 + 2 * s], sh[ww][4 + 2 * s], sh[ww][5 + 2 * s]);
+      }
+    for (int s = 0; s < kDiffVals; s++) part[(long long)blockIdx.x * kDiffVals + s] = sh[0][s];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_field_diff_final(const double *__restrict__ part, int nb,
+                                                         double *__restrict__ out) {
+  double mx[4] = {0.0, 0.0, 0.0, 0.0}, hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += 64)
+    for (int s = 0; s < 4; s++) {
+      mx[s] = nan_max(mx[s], part[(long long)b * kDiffVals + s]);
+      two_merge(hi[s], lo[s], part[(long long)b * kDiffVals + 4 + 2 * s], part[(long long)b * kDiffVals + 5 + 2 * s]);
+    }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int s = 0; s < 4; s++) {
+      mx[s] = nan_max(mx[s], __shfl_down(mx[s], off, 64));
+      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
+      two_merge(hi[s], lo[s], h2, l2);
+    }
+  if (threadIdx.x == 0)
+    for (int s = 0; s < 4; s++) {
+      out[s] = mx[s];
+      out[4 + s] = hi[s] + lo[s];
+    }
+}
+
+template <typename TA, typename TB>
+static void launch_field_diff(pn_ctx *a, pn_ctx *b, int nb) {
+  hipLaunchKernelGGL((k_field_diff<TA, TB>), dim3(nb), dim3(kRedThreads), 0, a->stream,
+                     (const TA *)a->phi[a->cur], (const TA *)a->pi, (const TB *)b->phi[b->cur],
+                     (const TB *)b->pi, a->n3, a->red_part);
+}
+
+extern "C" int pn_field_diff_get(pn_ctx *a, pn_ctx *b, pn_field_diff *out) {
+  if (!a || !b || !out) return pn_fail(a, PN_ERR_ARG, "pn_field_diff_get: NULL argument");
+  if (a->p.N != b->p.N) return pn_fail(a, PN_ERR_ARG, "pn_field_diff_get: N %d vs %d", a->p.N, b->p.N);
+  if (a->p.device != b->p.device)
+    return pn_fail(a, PN_ERR_ARG, "pn_field_diff_get: lattices on devices %d and %d", a->p.device, b->p.device);
+  PN_HIP(a, hipSetDevice(a->p.device));
+  PN_HIP(a, hipStreamSynchronize(b->stream));  // b's pending steps
+  const int nb = a->red_blocks;  // red_part holds 2 * kNSums >= kDiffVals values per block
+  static_assert(2 * kNSums >= kDiffVals, "partials fit the reduction scratch");
+  const bool a64 = a->p.precision == PN_FP64, b64 = b->p.precision == PN_FP64;
+  if (a64 && b64) launch_field_diff<double, double>(a, b, nb);
+  else if (!a64 && b64) launch_field_diff<float, double>(a, b, nb);
+  else if (a64 && !b64) launch_field_diff<double, float>(a, b, nb);
+  else launch_field_diff<float, float>(a, b, nb);
+  PN_CHECK_LAUNCH(a);
+  hipLaunchKernelGGL(k_field_diff_final, dim3(1), dim3(64), 0, a->stream, a->red_part, nb, a->red_out);
+  PN_CHECK_LAUNCH(a);
+  PN_HIP(a, hipMemcpyAsync(a->host_pin, a->red_out, 8 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+  PN_HIP(a, hipStreamSynchronize(a->stream));
+  const double *s = a->host_pin;
+  out->max_dphi = s[0];
+  out->max_dpi = s[1];
+  out->max_phi = s[2];
+  out->max_pi = s[3];
+  out->sum_dphi2 = s[4];
+  out->sum_phi2 = s[5];
+  out->sum_dpi2 = s[6];
+  out->sum_pi2 = s[7];
   return PN_OK;
 }

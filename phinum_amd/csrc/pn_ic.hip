@@ -168,18 +168,25 @@ extern "C" int pn_initial_conditions(pn_ctx *c, const pn_ic_params *ic) {
   const int N = c->p.N, H = N / 2 + 1;
   const double a = c->p.a;
   const size_t nm = (size_t)N * N * H;
-  std::vector<double> A(2 * nm), E(2 * nm);
-  int rc = pn_ic_modes(N, a, ic, A.data(), E.data());
-  if (rc != PN_OK) return pn_fail(c, rc, "%s", pn_last_error(nullptr));
+  // modes: drawn on the device (pn_icgen.hip) into dmodes[0..1], or on the
+  // host and uploaded one field at a time into dmodes[0]
+  std::vector<double> A, E;
+  if (!c->ic_device) {
+    A.resize(2 * nm);
+    E.resize(2 * nm);
+    int rc = pn_ic_modes(N, a, ic, A.data(), E.data());
+    if (rc != PN_OK) return pn_fail(c, rc, "%s", pn_last_error(nullptr));
+  }
 
   rocfft_plan plan = nullptr;
   rocfft_execution_info info = nullptr;
-  void *work = nullptr, *dmodes = nullptr, *dreal = nullptr;
+  void *work = nullptr, *dmodes[2] = {nullptr, nullptr}, *dreal = nullptr;
   auto cleanup = [&](int code) {
     if (plan) rocfft_plan_destroy(plan);
     if (info) rocfft_execution_info_destroy(info);
     (void)hipFree(work);
-    (void)hipFree(dmodes);
+    (void)hipFree(dmodes[0]);
+    (void)hipFree(dmodes[1]);
     (void)hipFree(dreal);
     return code;
   };
@@ -193,21 +200,28 @@ extern "C" int pn_initial_conditions(pn_ctx *c, const pn_ic_params *ic) {
     return cleanup(pn_fail(c, PN_ERR_FFT, "pn_initial_conditions: execution info failed"));
   hipError_t e = hipSuccess;
   if (wsize) e = hipMalloc(&work, wsize);
-  if (e == hipSuccess) e = hipMalloc(&dmodes, nm * 16);
+  if (e == hipSuccess) e = hipMalloc(&dmodes[0], nm * 16);
+  if (e == hipSuccess && c->ic_device) e = hipMalloc(&dmodes[1], nm * 16);
   if (e == hipSuccess) e = hipMalloc(&dreal, (size_t)c->n3 * 8);
   if (e != hipSuccess) return cleanup(pn_fail(c, PN_ERR_NOMEM, "pn_initial_conditions: %s", hipGetErrorString(e)));
+  if (c->ic_device) {
+    int rc = pn_icgen_modes(c, ic, dmodes[0], dmodes[1]);
+    if (rc != PN_OK) return cleanup(rc);
+  }
   if (wsize) rocfft_execution_info_set_work_buffer(info, work, wsize);
   rocfft_execution_info_set_stream(info, c->stream);
   const double norm = ((a * N) * (a * N)) * N * a;        // clusterfft.cpp:150
   const double normfactor = std::sqrt(((N * a) * (N * a)) * N * a);  // initialconditions.cpp:643
   dim3 grid((N + 31) / 32, (N + 31) / 32, N), block(32, 8);
   for (int f = 0; f < 2; f++) {
-    const std::vector<double> &src = f == 0 ? A : E;
     void *dst = f == 0 ? c->phi[c->cur] : c->pi;
     const double bg = f == 0 ? ic->phi0 : ic->pi0;
-    e = hipMemcpyAsync(dmodes, src.data(), nm * 16, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return cleanup(pn_fail(c, PN_ERR_HIP, "pn_initial_conditions: %s", hipGetErrorString(e)));
-    void *in[1] = {dmodes}, *out[1] = {dreal};
+    void *src = c->ic_device ? dmodes[f] : dmodes[0];
+    if (!c->ic_device) {
+      e = hipMemcpyAsync(src, (f == 0 ? A : E).data(), nm * 16, hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return cleanup(pn_fail(c, PN_ERR_HIP, "pn_initial_conditions: %s", hipGetErrorString(e)));
+    }
+    void *in[1] = {src}, *out[1] = {dreal};
     if (rocfft_execute(plan, in, out, info) != rocfft_status_success)
       return cleanup(pn_fail(c, PN_ERR_FFT, "pn_initial_conditions: c2r failed"));
     if (c->p.precision == PN_FP64)

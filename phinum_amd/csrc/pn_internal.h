@@ -33,6 +33,7 @@ struct pn_ctx {
   void *pi_alt;        // second Pi buffer for the two-step kernel (or NULL)
   int step2;           // pn_evolve advances pairs of steps in one pass
   int lds_max;         // LDS bytes a block may use on this device
+  int ic_device = 1;   // initial-condition modes drawn on the device (else on the host)
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch
@@ -52,6 +53,10 @@ struct pn_ctx {
 // builds a view context over existing device buffers (pn_batch.hip)
 int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, hipStream_t s,
                  pn_ctx **out);
+
+// initialize==3 modes of pn_ic_modes generated on the device (pn_icgen.hip):
+// dA, dE hold N*N*(N/2+1) double2 each; runs on c->stream and waits for it
+int pn_icgen_modes(pn_ctx *c, const pn_ic_params *ic, void *dA, void *dE);
 
 // error helpers ------------------------------------------------------------
 int pn_fail(pn_ctx *c, int code, const char *fmt, ...);

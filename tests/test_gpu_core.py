@@ -66,6 +66,25 @@ def test_two_step_kernel_bitwise_equals_single_steps(N, steps, prec):
         assert np.array_equal(_bits(out[0][0]), _bits(ophi)) and np.array_equal(_bits(out[0][1]), _bits(opi))
 
 
+@pytest.mark.parametrize("N,steps,prec", [(516, 3, pn.PN_FP64), (1024, 2, pn.PN_FP64), (1024, 2, pn.PN_FP32)])
+def test_two_step_kernel_bitwise_equals_single_steps_large(N, steps, prec):
+    # the N > 512 shapes, compared on the device: both runs start from the
+    # same initial condition and max |d| over both fields must be exactly 0
+    m = O.model(N)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    ic = pn.ic_params(5, a=m["a"], Lambda=m["Lambda"], phi0=float(np.sqrt(m["Lambda"])))
+    print("two-step vs single steps N=%d prec=%d" % (N, prec), flush=True)
+    with _lat(m, prec) as A, _lat(m, prec) as B:
+        assert A.two_step(True)
+        B.two_step(False)
+        for L in (A, B):
+            L.initial_conditions(ic)
+            L.evolve(dtk, dtd)
+        d = A.field_diff(B)
+    assert d["max_dphi"] == 0.0 and d["max_dpi"] == 0.0
+    assert d["sum_phi2"] > 0 and d["max_pi"] > 0
+
+
 def test_two_step_unavailable_is_reported():
     m = O.model(33)  # odd N: no vector width fits
     with _lat(m) as L:
