@@ -217,9 +217,10 @@ def batch_leg(pn, steps, out_every=500):
     """64 realizations: 32 x 64^3, 24 x 128^3, 8 x 192^3, seeds 0..63, lambda = 1,
     g drawn uniformly from [0.1, 0.4] with numpy's default_rng(0); FP64, one
     launch per pair of steps for all of them (two-step tiles), 100-step HIP
-    graphs, tiles autotuned per size class.  Energies and the false-vacuum
-    check every `out_every` steps (outside the timing).  Returns a dict for the
-    JSON line."""
+    graphs, tiles autotuned per size class.  Every `out_every` steps (outside
+    the timing) each realization gets its energy, the false-vacuum check, its
+    shell spectrum and its island catalog (energy density above 4x its mean,
+    islands.cpp:783-786, 6-connected).  Returns a dict for the JSON line."""
     sizes = [64] * 32 + [128] * 24 + [192] * 8
     g = np.random.default_rng(0).uniform(0.1, 0.4, len(sizes))
     reals = [dict(N=N, a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1,
@@ -233,25 +234,38 @@ def batch_leg(pn, steps, out_every=500):
         e0 = np.array([B.view(r).volume_averages().energy for r in range(len(sizes))])
         decayed = np.zeros(len(sizes), dtype=np.int32)
         total = 0.0
+        t_analysis = 0.0
+        n_out = 0
         s = 0
         while s < steps:
             n = min(out_every, steps - s)
             total += B.evolve_timed(dtk[s:s + n], dtd[s:s + n], use_graph=True)
             s += n
+            t0 = time.time()
             av = [B.view(r).volume_averages() for r in range(len(sizes))]
             # false_vacuum_decayed (fieldanalysis.cpp:887-893) per realization
             newly = np.array([(a.interaction < 0 or a.interaction != a.interaction) for a in av])
             if np.any(newly & (decayed == 0)):
                 decayed |= newly.astype(np.int32)
                 B.set_decayed(decayed)
+            spectra = [B.view(r).spectrum() for r in range(len(sizes))]
+            cats = [B.view(r).islands(1, 4.0 * av[r].energy)[0] for r in range(len(sizes))]
+            t_analysis += time.time() - t0
+            n_out += 1
         e1 = np.array([a.energy for a in av])
+        isl = [len(c) for c in cats]
+        vmax = [max((int(i.nn) for i in c), default=0) for c in cats]
+        pars = [float(sp["F"].sum()) for sp in spectra]
     sites = float(sum(N ** 3 for N in sizes))
     return {"value": sites * steps / (total * 1e-3), "unit": "site-updates/s",
             "realizations": len(sizes), "sites": int(sites), "steps": steps,
             "ms_per_step": total / steps, "two_step": two, "tiles_rows_zblock": tiles,
             "roofline_frac": 32.0 * sites / (total / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "decayed": int(decayed.sum()),
-            "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0)))}
+            "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0))),
+            "outputs": n_out, "analysis_s_per_output": t_analysis / max(n_out, 1),
+            "final_islands_per_realization": isl, "final_largest_island_sites": vmax,
+            "final_spectrum_total_power": [float("%.6g" % v) for v in pars]}
 
 
 def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, check_steps=1000, eps=1e-6):

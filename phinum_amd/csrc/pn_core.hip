@@ -201,7 +201,7 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
 // tile of RT full x rows (N/W threads of W sites each) and ZB planes; the
 // tile's work is step2_tile.  Needs N % W == 0, (N/W)*RT <= 1024 and
 // 4*RT*N values of LDS (2*RT*N without the Phi planes, LP false).
-template <typename T, typename C, bool POW2, int W, int RT, bool LP = true>
+template <typename T, typename C, bool POW2, int W, int RT, bool LP = true, int FAST = DIV_FAST>
 __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const T *__restrict__ pi,
                                                 T *__restrict__ pio, T *__restrict__ phin, int N,
                                                 int ZB, C k, T dk0, T dd0, T dk1, T dd1) {
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int yt = tile % YT, zt = tile / YT;
   const int z0 = zt * ZB;
-  step2_tile<T, C, POW2, W, true, LP>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+  step2_tile<T, C, POW2, W, true, LP, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
                             (int)threadIdx.y, true, yt * (RT - 2), z0, min(z0 + ZB, N), k, dk0, dd0,
                             dk1, dd1);
 }
@@ -333,7 +333,7 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
 }
 
-template <typename T, typename C, bool POW2, int W, int RT, bool LP = true>
+template <typename T, typename C, bool POW2, int W, int RT, bool LP = true, int FAST = DIV_FAST>
 static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T dd1) {
   const int N = c->p.N;
   const int YT = (N + RT - 3) / (RT - 2), ZT = (N + ZB - 1) / ZB;
@@ -342,30 +342,40 @@ static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T
     // beyond 64 KB the launch needs the kernel's dynamic LDS limit raised (once)
     static int raised = 0;
     if (!raised) {
-      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT, LP>,
+      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT, LP, FAST>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
       raised = 1;
     }
   }
-  hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT, LP>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
+  hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT, LP, FAST>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
+}
+
+template <int FAST>
+static void launch_step2_f64(pn_ctx *c, const CoefF64 &k, const Step2Shape &s, const double *dtk,
+                             const double *dtd) {
+  const double a0 = dtk[0], b0 = dtd[0], a1 = dtk[1], b1 = dtd[1];
+  if (s.W == 2) {
+    if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 2, 6, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+    else launch_step2_t<double, CoefF64, false, 2, 6, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+  } else if (s.RT == 6) {
+    if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 6, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+    else launch_step2_t<double, CoefF64, false, 4, 6, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+  } else {
+    if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 4, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+    else launch_step2_t<double, CoefF64, false, 4, 4, true, FAST>(c, k, s.ZB, a0, b0, a1, b1);
+  }
 }
 
 // steps n and n+1 in one pass; Pi moves to the second buffer
 static int launch_step2(pn_ctx *c, const Step2Shape &s, const double *dtk, const double *dtd) {
   if (c->p.precision == PN_FP64) {
     CoefF64 k = coef64(c);
-    if (s.W == 2) {
-      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-      else launch_step2_t<double, CoefF64, false, 2, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-    } else if (s.RT == 6) {
-      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-      else launch_step2_t<double, CoefF64, false, 4, 6>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-    } else {
-      if (c->k.a2_pow2) launch_step2_t<double, CoefF64, true, 4, 4>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-      else launch_step2_t<double, CoefF64, false, 4, 4>(c, k, s.ZB, dtk[0], dtd[0], dtk[1], dtd[1]);
-    }
+    // q/kappa in two operations, or with the correction step: a separate
+    // kernel each, so neither carries the other's instructions
+    if (k.rk_bmr) launch_step2_f64<DIV_FAST_2OP>(c, k, s, dtk, dtd);
+    else launch_step2_f64<DIV_FAST_KC>(c, k, s, dtk, dtd);
   } else {
     CoefF32 k = coef32(c);
     const float a0 = (float)dtk[0], b0 = (float)dtd[0], a1 = (float)dtk[1], b1 = (float)dtd[1];

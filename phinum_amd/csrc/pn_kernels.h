@@ -47,11 +47,16 @@ __device__ __forceinline__ bool fast_div_site(double f) {
   return hi - 0x39B00000u < 0x0C800000u;
 }
 
-enum { DIV_IEEE = 0, DIV_FAST = 1, DIV_PER_SITE = 2 };
+// DIV_FAST: the fast divisions, with the correction step for q/kappa chosen
+// at run time from k.rk_bmr; DIV_FAST_2OP / DIV_FAST_KC: the same with that
+// choice made at compile time (q/kappa in two operations / corrected);
+// DIV_PER_SITE: DIV_FAST or IEEE, decided per site.
+enum { DIV_IEEE = 0, DIV_FAST = 1, DIV_PER_SITE = 2, DIV_FAST_2OP = 3, DIV_FAST_KC = 4 };
 
 // Theory-0 update of one site, reference order (dynamics.cpp:80,84,183).
-// DIV_FAST needs k.fdiv and fast_div_site(f), checked by the caller (the
-// step kernels check a whole wave at once); DIV_PER_SITE checks here.
+// DIV_FAST and DIV_FAST_KC need k.fdiv and fast_div_site(f), checked by the
+// caller (the step kernels check a whole wave at once); DIV_PER_SITE checks
+// here.
 template <bool POW2, int DIV>
 __device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, double xm, double yp,
                                                 double ym, double zp, double zm, double f,
@@ -65,11 +70,12 @@ __device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, dou
   // the reference's 0.0 + f*f: f*f is never -0, so adding +0 changes no bit
   const double q = f * f;
   double B, C;
-  const bool fast = DIV == DIV_FAST || (DIV == DIV_PER_SITE && k.fdiv && fast_div_site(f));
+  const bool fast = DIV != DIV_IEEE && (DIV != DIV_PER_SITE || (k.fdiv && fast_div_site(f)));
   if (fast) {
     B = k.lmode * pn_div_bmr(q, k.r6);
     double t = pn_div_bmr(q, k.rk);
-    if (!k.rk_bmr) t = pn_div_correct(q, k.rk, t);
+    if (DIV == DIV_FAST_KC || ((DIV == DIV_FAST || DIV == DIV_PER_SITE) && !k.rk_bmr))
+      t = pn_div_correct(q, k.rk, t);
     C = pn_div_bmr(t * t, k.r120);
   } else {
     B = k.lmode * (q / 6.);
@@ -79,12 +85,13 @@ __device__ __forceinline__ void site_update_f64(const CoefF64 &k, double xp, dou
   p = p + (lap - ((k.c2 + B) + C) * f) * dtk;
 }
 
-// Runs body(std::integral_constant<int, DIV_FAST>) when every active lane of
-// the wave passes `lane_ok` and the coefficients allow it, else the IEEE form:
-// one wave-uniform branch instead of a divergent one per site.
-template <typename F>
+// Runs body(std::integral_constant<int, FAST>) (FAST one of the DIV_FAST
+// modes) when every active lane of the wave passes `lane_ok` and the
+// coefficients allow it, else the IEEE form: one wave-uniform branch instead
+// of a divergent one per site.
+template <int FAST = DIV_FAST, typename F>
 __device__ __forceinline__ void with_div_mode(const CoefF64 &k, bool lane_ok, F &&body) {
-  if (k.fdiv && __all(lane_ok)) body(std::integral_constant<int, DIV_FAST>());
+  if (k.fdiv && __all(lane_ok)) body(std::integral_constant<int, FAST>());
   else body(std::integral_constant<int, DIV_IEEE>());
 }
 
@@ -148,7 +155,7 @@ struct RedCoef {
 // tile's Phi planes in LDS too (two more planes of RT*N values after the
 // intermediate ones): stage A then reads its x and y neighbours there instead
 // of from L2, except the one outside row of each edge row.
-template <typename T, typename C, bool POW2, int W, bool PF = true, bool LP = true>
+template <typename T, typename C, bool POW2, int W, bool PF = true, bool LP = true, int FAST = DIV_FAST>
 __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *__restrict__ pi,
                                            T *__restrict__ pio, T *__restrict__ phin, T *lds_g, int N,
                                            int RT, int tx, int ty, bool active, int y0, int z0,
@@ -235,7 +242,7 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
         bool ok = true;
 #pragma unroll
         for (int j = 0; j < W; j++) ok = ok && fast_div_site(an_c[j]);
-        with_div_mode(k, ok, stage_a);
+        with_div_mode<FAST>(k, ok, stage_a);
       } else {
         stage_a(std::integral_constant<int, DIV_IEEE>());
       }
@@ -267,7 +274,7 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
         bool ok = true;
 #pragma unroll
         for (int j = 0; j < W; j++) ok = ok && fast_div_site(b_c[j]);
-        with_div_mode(k, ok, stage_b);
+        with_div_mode<FAST>(k, ok, stage_b);
       } else {
         stage_b(std::integral_constant<int, DIV_IEEE>());
       }
