@@ -364,6 +364,8 @@ def main():
     ap.add_argument("--precision-size", type=int, default=1024)
     ap.add_argument("--precision-steps", type=int, default=10000)
     ap.add_argument("--precision-every", type=int, default=1000)
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="only the FP32 form at --size for --steps steps (for profiling)")
     ap.add_argument("--precision-only", action="store_true",
                     help="only the precision form (FP32 against FP64 at --precision-size)")
     ap.add_argument("--single-step", action="store_true", help="time the one-step kernel only")
@@ -397,13 +399,17 @@ def main():
     N, K, W = args.size, args.steps, args.warmup
     two = not args.single_step
     m = model()
+    if args.fp32_only:
+        ms32, _, on32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
+        print(json.dumps({"fp32_ms_per_step": ms32 / K, "two_step_kernel": on32}), flush=True)
+        return
     log("fp64 %d^3: %d warmup + %d timed steps (two-step kernel %s)" % (N, W, K, "on" if two else "off"))
     t0 = time.time()
     ms64, e64, on = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m, two_step=two)
     log("fp64 done in %.1fs wall, %.4f ms/step" % (time.time() - t0, ms64 / K))
     sites = float(N) ** 3
     value = sites * K / (ms64 * 1e-3)
-    kname = ("k_step2<double, CoefF64, POW2, W, RT> (two leapfrog steps per pass)" if on else
+    kname = ("k_step2<double, CoefF64, POW2, W, RT, LP, FAST> (two leapfrog steps per pass)" if on else
              "k_step_vec<double, CoefF64, POW2, 2, 4, ZB, 0> (one fused step)")
     prof = ("k_step2_fp64_%dcube" if on else "k_step_vec_fp64_%dcube") % N
     line = {
@@ -441,15 +447,15 @@ def main():
             "steps_with_dE_over_E_le_1e-6": int(K if over.size == 0 else (over[0] - 1) * args.out_every),
             "dE_over_E_vs_fp64_by_output": [float("%.3g" % r) for r in rel[::max(1, len(rel) // 10)]],
             "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
-            "roofline": roofline("k_step2<float, CoefF32, true, 4, 8>", N, ms32, K, args.out_every,
-                                 on32, 4),
+            "roofline": roofline("k_step2<float, CoefF32, true, W, RT, LP, FAST>", N, ms32, K, args.out_every,
+                                 on32, 4, "k_step2_fp32_%dcube" % N),
         }
         N5 = 512
         log("fp64 %d^3" % N5)
         ms5, _, on5 = run_leg(pn, N5, pn.PN_FP64, 200, 20, 200, 5, m)
         line["fp64_512"] = {"ms_per_step": ms5 / 200,
                             "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
-                            "roofline": roofline("k_step2<double, CoefF64, true, 4, 6>", N5, ms5, 200,
+                            "roofline": roofline("k_step2<double, CoefF64, true, 4, 6, true, 3>", N5, ms5, 200,
                                                  200, on5, 8, "k_step2_fp64_512cube")}
         log("precision form %d^3" % args.precision_size)
         line["precision_%d" % args.precision_size] = precision_leg(
