@@ -249,12 +249,12 @@ def batch_leg(pn, steps, out_every=500):
                 decayed |= newly.astype(np.int32)
                 B.set_decayed(decayed)
             spectra = [B.view(r).spectrum() for r in range(len(sizes))]
-            cats = [B.view(r).islands(1, 4.0 * av[r].energy)[0] for r in range(len(sizes))]
+            cats = [B.view(r).islands(1, 4.0 * av[r].energy, as_array=True)[0] for r in range(len(sizes))]
             t_analysis += time.time() - t0
             n_out += 1
         e1 = np.array([a.energy for a in av])
         isl = [len(c) for c in cats]
-        vmax = [max((int(i.nn) for i in c), default=0) for c in cats]
+        vmax = [int(c["nn"].max()) if len(c) else 0 for c in cats]
         pars = [float(sp["F"].sum()) for sp in spectra]
     sites = float(sum(N ** 3 for N in sizes))
     return {"value": sites * steps / (total * 1e-3), "unit": "site-updates/s",

@@ -322,14 +322,27 @@ class Lattice:
         self._chk(self.L.pn_fft_modes(self.h, int(field), _dptr(out)))
         return out.view(np.complex128).reshape(self.N, self.N, H)
 
-    def islands(self, mode, thr, capacity=65536, labels=False):
-        arr = (PnIsland * max(capacity, 1))()
-        lab = np.empty(self.N ** 3, dtype=np.int32) if labels else None
-        cat = PnCatalog(capacity, 0, 0.0, arr,
-                        lab.ctypes.data_as(C.POINTER(C.c_int)) if labels else None)
-        self._chk(self.L.pn_islands(self.h, int(mode), float(thr), C.byref(cat)))
-        rows = [arr[i] for i in range(cat.count)]
-        return rows, lab, cat.threshold
+    def islands(self, mode, thr, capacity=65536, labels=False, grow=True, as_array=False):
+        """Island catalog (rows of PnIsland, or with as_array one numpy
+        structured array over the same memory), the label array if asked for,
+        and the threshold used.  With grow, a catalog larger than `capacity`
+        is fetched again with room for all of it (the library reports the
+        count with PN_ERR_CAPACITY); without, that error is raised."""
+        while True:
+            arr = (PnIsland * max(capacity, 1))()
+            lab = np.empty(self.N ** 3, dtype=np.int32) if labels else None
+            cat = PnCatalog(capacity, 0, 0.0, arr,
+                            lab.ctypes.data_as(C.POINTER(C.c_int)) if labels else None)
+            rc = self.L.pn_islands(self.h, int(mode), float(thr), C.byref(cat))
+            if rc == PN_ERR_CAPACITY and grow and cat.count > capacity:
+                capacity = cat.count
+                continue
+            self._chk(rc)
+            if as_array:
+                rows = np.ctypeslib.as_array(arr)[:cat.count]
+            else:
+                rows = [arr[i] for i in range(cat.count)]
+            return rows, lab, cat.threshold
 
     def initial_conditions(self, ic):
         """SetInitialConditions (initialize==3) with a PnIcParams."""

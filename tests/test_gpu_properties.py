@@ -96,3 +96,21 @@ def test_islands_128_cube_match_bfs():
     nn = np.array([r.nn for r in rows])
     assert np.array_equal(nn, cat[:, 1].astype(np.int64))
     assert np.array_equal(np.array([r.peak_pos for r in rows]), cat[:, 3].astype(np.int64))
+
+
+def test_island_catalog_capacity_error_and_growth():
+    # a catalog larger than the caller's array: PN_ERR_CAPACITY with the
+    # count reported, and the wrapper's retry returns the whole catalog
+    m = O.model(32)
+    rng = np.random.default_rng(4)
+    with _lat(m) as L:
+        L.set_fields(rng.standard_normal(32 ** 3), rng.standard_normal(32 ** 3))
+        full, _, _ = L.islands(0, 1.5, capacity=1 << 16)
+        assert len(full) > 8
+        with pytest.raises(pn.PhiNumError) as e:
+            L.islands(0, 1.5, capacity=8, grow=False)
+        assert e.value.code == pn.PN_ERR_CAPACITY
+        grown, _, _ = L.islands(0, 1.5, capacity=8)
+        arr, _, _ = L.islands(0, 1.5, capacity=8, as_array=True)
+    assert [(r.index, r.nn, r.peak_pos) for r in grown] == [(r.index, r.nn, r.peak_pos) for r in full]
+    assert arr["nn"].tolist() == [r.nn for r in full] and arr["peak_pos"].tolist() == [r.peak_pos for r in full]
