@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pn_internal.h"
+#include "pn_kernels.h"
 
 struct pn_fft_state {
   rocfft_plan plan = nullptr;
@@ -64,8 +65,15 @@ extern "C" int pn_spectrum_nbins(int N, double a) {
   return (int)(pTmax / pTmin + 0.5) + 2;             // :47
 }
 
-static constexpr int kBinThreads = 256;
 static constexpr int kBinBlocks = 512;
+
+// Waves per binning block.  Each wave keeps its own compensated histogram,
+// (hi, lo) of the three sums per bin (48 bytes), and the block one int count
+// per bin; up to four waves within 64 KB of LDS.
+static int bin_waves(int nbins) {
+  const long long room = (65536LL - 4LL * nbins) / (48LL * nbins);
+  return room < 1 ? 1 : (room > 4 ? 4 : (int)room);
+}
 
 static int ensure_fft(pn_ctx *c) {
   if (c->fft) return PN_OK;
@@ -119,7 +127,7 @@ static int ensure_fft(pn_ctx *c) {
   f->nbins = pn_spectrum_nbins(N, a);
   f->binsize = (2.0 / (a)) * sin(acos(-1.) / N);
   f->nblocks = kBinBlocks;
-  PN_HIP(c, hipMalloc(&f->bin_part, (size_t)f->nblocks * f->nbins * 3 * sizeof(double)));
+  PN_HIP(c, hipMalloc(&f->bin_part, (size_t)f->nblocks * f->nbins * 6 * sizeof(double)));
   PN_HIP(c, hipMalloc(&f->cnt_part, (size_t)f->nblocks * f->nbins * sizeof(int)));
   PN_HIP(c, hipMalloc(&f->bin_out, (size_t)f->nbins * 3 * sizeof(double)));
   PN_HIP(c, hipMalloc(&f->cnt_out, (size_t)f->nbins * sizeof(int)));
@@ -144,49 +152,82 @@ __global__ __launch_bounds__(256) void k_transpose_xy(const T *__restrict__ in, 
   }
 }
 
-// shell binning of momentumspace.cpp:196-247; LDS histogram per block,
-// per-block partials merged in a fixed order by k_bin_final
+// Shell binning of momentumspace.cpp:196-247 with compensated, deterministic
+// FP64 sums.  A wave takes one (pz, px) row of modes at a time, py = lane,
+// lane + 64, ...: mode m = py + H*(pz*N + px).  Along a row the bin index
+// never decreases (psqr grows with py and every operation on it is monotone),
+// so equal bins are contiguous lanes.  A segmented scan over shuffles sums
+// each run in (hi, lo) form, and the run's last lane adds it to the wave's own
+// LDS histogram: no two lanes touch one bin at once and the order of the
+// additions is fixed by the launch.  The waves' histograms are merged in wave
+// order, the blocks' in block order by k_bin_final.  Counts are integers
+// (atomic adds are exact).
 template <typename T2>
-__global__ __launch_bounds__(kBinThreads) void k_bin(const T2 *__restrict__ cphi,
-                                                     const T2 *__restrict__ cpi,
-                                                     const double *__restrict__ tab, int N,
-                                                     double fac, double binsize, int nbins,
-                                                     double *__restrict__ part,
-                                                     int *__restrict__ cpart) {
+__global__ __launch_bounds__(256) void k_bin(const T2 *__restrict__ cphi, const T2 *__restrict__ cpi,
+                                             const double *__restrict__ tab, int N, double fac,
+                                             double binsize, int nbins, double *__restrict__ part,
+                                             int *__restrict__ cpart) {
   extern __shared__ double lds[];
-  double *sf = lds, *sk = lds + nbins, *sq = lds + 2 * nbins;
-  int *sc = (int *)(lds + 3 * nbins);
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) { sf[b] = 0; sk[b] = 0; sq[b] = 0; sc[b] = 0; }
+  const int nw = (int)(blockDim.x >> 6), w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  double *hist = lds + (size_t)w * nbins * 6;  // per bin: F hi, F lo, K hi, K lo, Q hi, Q lo
+  int *sc = (int *)(lds + (size_t)nw * nbins * 6);
+  for (int i = threadIdx.x; i < nw * nbins * 6; i += blockDim.x) lds[i] = 0.0;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) sc[b] = 0;
   __syncthreads();
   const int H = N / 2 + 1;
-  const long long nm = (long long)N * N * H;
   const double *tx = tab, *ty = tab + N, *tz = tab + N + H;
-  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm;
-       m += (long long)gridDim.x * blockDim.x) {
-    const int py = (int)(m % H);
-    const int px = (int)((m / H) % N);
-    const int pz = (int)(m / ((long long)H * N));
-    const double psqr = tx[px] + ty[py] + tz[pz];
-    const int b = (int)(sqrt(psqr) / binsize + 0.5);
-    if (b >= nbins) continue;  // cannot happen: nbins covers sqrt(3)*pTmax
-    const T2 u = cphi[m], v = cpi[m];
-    const double pr = fac * (double)u.x, pim = fac * (double)u.y;
-    const double kr = fac * (double)v.x, kim = fac * (double)v.y;
-    const double ff = pr * pr + pim * pim, kk = kr * kr + kim * kim, qq = pr * kr + pim * kim;
-    const int w = (py != 0 && py != N / 2) ? 2 : 1;
-    atomicAdd(&sf[b], w == 2 ? ff + ff : ff);
-    atomicAdd(&sk[b], w == 2 ? kk + kk : kk);
-    atomicAdd(&sq[b], w == 2 ? qq + qq : qq);
-    atomicAdd(&sc[b], w);
+  const long long rows = (long long)N * N;
+  for (long long r = (long long)blockIdx.x * nw + w; r < rows; r += (long long)gridDim.x * nw) {
+    const int px = (int)(r % N), pz = (int)(r / N);
+    const long long m0 = r * H;
+    for (int p0 = 0; p0 < H; p0 += 64) {
+      const int py = p0 + lane;
+      int b = nbins;  // lanes past the row end: a run of their own, never added
+      double hi[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+      if (py < H) {
+        const double psqr = tx[px] + ty[py] + tz[pz];
+        b = (int)(sqrt(psqr) / binsize + 0.5);
+        if (b > nbins) b = nbins;  // cannot happen: nbins covers sqrt(3)*pTmax
+        const T2 u = cphi[m0 + py], v = cpi[m0 + py];
+        const double pr = fac * (double)u.x, pim = fac * (double)u.y;
+        const double kr = fac * (double)v.x, kim = fac * (double)v.y;
+        const double ff = pr * pr + pim * pim, kk = kr * kr + kim * kim, qq = pr * kr + pim * kim;
+        const bool twice = py != 0 && py != N / 2;  // the py -> N-py half not stored
+        hi[0] = twice ? ff + ff : ff;
+        hi[1] = twice ? kk + kk : kk;
+        hi[2] = twice ? qq + qq : qq;
+        if (b < nbins) atomicAdd(&sc[b], twice ? 2 : 1);
+      }
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int bo = __shfl_up(b, d, 64);
+        double h2[3], l2[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          h2[j] = __shfl_up(hi[j], d, 64);
+          l2[j] = __shfl_up(lo[j], d, 64);
+        }
+        if (lane >= d && bo == b)
+#pragma unroll
+          for (int j = 0; j < 3; j++) two_merge(hi[j], lo[j], h2[j], l2[j]);
+      }
+      const int bn = __shfl_down(b, 1, 64);
+      if (b < nbins && (lane == 63 || bn != b))
+#pragma unroll
+        for (int j = 0; j < 3; j++) two_merge(hist[6 * b + 2 * j], hist[6 * b + 2 * j + 1], hi[j], lo[j]);
+    }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-    const long long o = (long long)blockIdx.x * nbins + b;
-    part[3 * o + 0] = sf[b];
-    part[3 * o + 1] = sk[b];
-    part[3 * o + 2] = sq[b];
-    cpart[o] = sc[b];
+  for (int i = threadIdx.x; i < nbins * 3; i += blockDim.x) {
+    const int b = i / 3, j = i % 3;
+    double h = lds[6 * b + 2 * j], l = lds[6 * b + 2 * j + 1];
+    for (int ww = 1; ww < nw; ww++)
+      two_merge(h, l, lds[(size_t)ww * nbins * 6 + 6 * b + 2 * j], lds[(size_t)ww * nbins * 6 + 6 * b + 2 * j + 1]);
+    const long long o = ((long long)blockIdx.x * nbins + b) * 6 + 2 * j;
+    part[o] = h;
+    part[o + 1] = l;
   }
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) cpart[(long long)blockIdx.x * nbins + b] = sc[b];
 }
 
 __global__ void k_bin_final(const double *__restrict__ part, const int *__restrict__ cpart, int nb,
@@ -196,14 +237,9 @@ __global__ void k_bin_final(const double *__restrict__ part, const int *__restri
   double hi[3] = {0, 0, 0}, lo[3] = {0, 0, 0};
   int cnt = 0;
   for (int k = 0; k < nb; k++) {
-    const long long o = (long long)k * nbins + b;
-    for (int j = 0; j < 3; j++) {
-      const double x = part[3 * o + j];
-      const double s = hi[j] + x, bp = s - hi[j];
-      lo[j] += (hi[j] - (s - bp)) + (x - bp);
-      hi[j] = s;
-    }
-    cnt += cpart[o];
+    const long long o = ((long long)k * nbins + b) * 6;
+    for (int j = 0; j < 3; j++) two_merge(hi[j], lo[j], part[o + 2 * j], part[o + 2 * j + 1]);
+    cnt += cpart[(long long)k * nbins + b];
   }
   for (int j = 0; j < 3; j++) out[3 * b + j] = hi[j] + lo[j];
   cout[b] = cnt;
@@ -247,13 +283,15 @@ extern "C" int pn_spectrum_get(pn_ctx *c, pn_spectrum *out) {
     return pn_fail(c, PN_ERR_CAPACITY, "pn_spectrum_get: nbins %d < %d", out->nbins, f->nbins);
   const double a = c->p.a;
   const double fac = (a * a) * a;  // Factor = sqr(a_t)*a_eta (clusterfft.cpp:187)
-  const size_t lds = (size_t)f->nbins * (3 * sizeof(double) + sizeof(int));
+  const int nw = bin_waves(f->nbins);
+  const size_t lds = (size_t)nw * f->nbins * 6 * sizeof(double) + (size_t)f->nbins * sizeof(int);
+  if (lds > 65536) return pn_fail(c, PN_ERR_ARG, "pn_spectrum_get: %d bins exceed the binning kernel's LDS", f->nbins);
   if (c->p.precision == PN_FP64)
-    hipLaunchKernelGGL(k_bin<double2>, dim3(f->nblocks), dim3(kBinThreads), lds, c->stream,
+    hipLaunchKernelGGL(k_bin<double2>, dim3(f->nblocks), dim3(64 * nw), lds, c->stream,
                        (const double2 *)f->cphi, (const double2 *)f->cpi, f->tab, c->p.N, fac,
                        f->binsize, f->nbins, f->bin_part, f->cnt_part);
   else
-    hipLaunchKernelGGL(k_bin<float2>, dim3(f->nblocks), dim3(kBinThreads), lds, c->stream,
+    hipLaunchKernelGGL(k_bin<float2>, dim3(f->nblocks), dim3(64 * nw), lds, c->stream,
                        (const float2 *)f->cphi, (const float2 *)f->cpi, f->tab, c->p.N, fac,
                        f->binsize, f->nbins, f->bin_part, f->cnt_part);
   PN_CHECK_LAUNCH(c);
