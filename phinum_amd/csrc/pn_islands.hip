@@ -13,16 +13,21 @@
 //   3. flatten     : label[i] = root
 //   4. numbering   : roots in increasing index order get 0,1,2,... -- exactly
 //                    the order in which the raster scan meets the islands
-//   5. statistics  : integer counts and offset sums exactly, FP64 energy sums,
-//                    the peak as the largest key (phi^2 or Delta) with ties
-//                    broken towards the smallest index.
+//   5. statistics  : integer counts and offset sums exactly, the peak as the
+//                    largest key (phi^2 or Delta) with ties broken towards the
+//                    smallest index, and the FP64 sums (phi, potential,
+//                    kinetic and gradient energy) over each island's sites in
+//                    raster order (a stable sort groups them), compensated and
+//                    merged in a fixed order: the same bits on every call.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "pn_internal.h"
+#include "pn_kernels.h"
 
 struct pn_island_state {
   int *label = nullptr;      // N^3
@@ -34,6 +39,14 @@ struct pn_island_state {
   long long *iint = nullptr;           // per island: nn, first, peak, ties, sdx, sdy, sdz
   double *idbl = nullptr;              // per island: sum phi, pot, kin, grad, peak value
   int *count = nullptr;
+  // the island sites grouped by island for the FP sums: raster-order
+  // compaction, then a stable sort by island number
+  int *mblk = nullptr;                 // per-block island-site counts / offsets
+  long long mcap = 0;                  // allocated island-site entries
+  int *mkey = nullptr, *mkey2 = nullptr, *msite = nullptr, *msite2 = nullptr;
+  int *ist = nullptr;                  // per island: first entry in the sorted sites (+ total)
+  void *sort_tmp = nullptr;
+  size_t sort_bytes = 0;
 };
 
 void pn_free_islands(pn_ctx *c) {
@@ -46,6 +59,13 @@ void pn_free_islands(pn_ctx *c) {
   hipFree(s->iint);
   hipFree(s->idbl);
   hipFree(s->count);
+  hipFree(s->mblk);
+  hipFree(s->mkey);
+  hipFree(s->mkey2);
+  hipFree(s->msite);
+  hipFree(s->msite2);
+  hipFree(s->ist);
+  hipFree(s->sort_tmp);
   delete s;
   c->isl = nullptr;
 }
@@ -236,7 +256,7 @@ struct IslStatCoef {
   double a2, cm, lmode, d6;
 };
 
-// pass A: sizes, first site, sums, max key
+// pass A: sizes and max key (integer atomics: exact in any order)
 template <typename T>
 __global__ __launch_bounds__(kT) void k_isl_stats(const T *__restrict__ phi, const T *__restrict__ pi,
                                                   int N, IslCoef k, const int *__restrict__ label,
@@ -251,24 +271,112 @@ __global__ __launch_bounds__(kT) void k_isl_stats(const T *__restrict__ phi, con
     if (r < 0) continue;
     const int id = cid[r];
     const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    const double f = (double)phi[i], p = (double)pi[i];
     double key;
-    if (k.mode == 0) key = f * f;
+    if (k.mode == 0) key = (double)phi[i] * (double)phi[i];
     else key = site_delta(phi, pi, i, x, y, z, N, k);
-    const double fx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)];
-    const double fy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N];
-    const double fz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P];
-    // islands.cpp:131-142
-    const double grad = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / k.a2;
-    const double kin = 0.5 * (p * p);
-    const double q2 = (f * f) * (f * f);
-    const double pot = k.cm * (f * f) + k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
     atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 0], 1ULL);
-    atomicAdd(&idbl[(long long)id * kND + 0], f);
-    atomicAdd(&idbl[(long long)id * kND + 1], pot);
-    atomicAdd(&idbl[(long long)id * kND + 2], kin);
-    atomicAdd(&idbl[(long long)id * kND + 3], grad);
     atomicMax(&ikey[id], ord_key(key));
+  }
+}
+
+// island sites per chunk, for the raster-order compaction below
+__global__ __launch_bounds__(kT) void k_isl_mcount(const int *__restrict__ label, long long n3,
+                                                   int *__restrict__ mblk) {
+  __shared__ int s[kT];
+  const long long base = (long long)blockIdx.x * kChunk;
+  int cnt = 0;
+  for (int j = threadIdx.x; j < kChunk; j += kT) {
+    const long long i = base + j;
+    if (i < n3 && label[i] >= 0) cnt++;
+  }
+  s[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mblk[blockIdx.x] = s[0];
+}
+
+// island sites in raster order, each with its island number
+__global__ __launch_bounds__(kT) void k_isl_mscatter(const int *__restrict__ label, long long n3,
+                                                     const int *__restrict__ mblk,
+                                                     const int *__restrict__ cid,
+                                                     int *__restrict__ mkey, int *__restrict__ msite) {
+  __shared__ int s[kT];
+  __shared__ int carry;
+  const long long base = (long long)blockIdx.x * kChunk;
+  if (threadIdx.x == 0) carry = mblk[blockIdx.x];
+  __syncthreads();
+  for (int j0 = 0; j0 < kChunk; j0 += kT) {
+    const long long i = base + j0 + threadIdx.x;
+    const int r = i < n3 ? label[i] : -1;
+    const int f = r >= 0 ? 1 : 0;
+    s[threadIdx.x] = f;
+    __syncthreads();
+    for (int o = 1; o < kT; o <<= 1) {
+      const int t = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (f) {
+      const int pos = carry + s[threadIdx.x] - 1;
+      mkey[pos] = cid[r];
+      msite[pos] = (int)i;
+    }
+    __syncthreads();
+    if (threadIdx.x == kT - 1) carry += s[kT - 1];
+    __syncthreads();
+  }
+}
+
+// first entry of each island among the sorted sites; ist[nisl] = total
+__global__ void k_isl_starts(const int *__restrict__ key, int m, int nisl, int *__restrict__ ist) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  if (p == 0 || key[p - 1] != key[p]) ist[key[p]] = p;
+  if (p == m - 1) ist[nisl] = m;
+}
+
+// FP64 sums of an island over its sites in raster order (islands.cpp:131-142:
+// phi, potential, kinetic and gradient energy), one wave per island: lane
+// sums in (hi, lo) form merged in a fixed tree
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_sums(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                 int N, IslCoef k, const int *__restrict__ ist,
+                                                 const int *__restrict__ site, int nisl,
+                                                 double *__restrict__ idbl) {
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  const long long P = (long long)N * N;
+  for (int id = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); id < nisl; id += (int)gridDim.x * wpb) {
+    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    const int e1 = ist[id + 1];
+    for (int e = ist[id] + lane; e < e1; e += 64) {
+      const long long i = site[e];
+      const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+      const double f = (double)phi[i], p = (double)pi[i];
+      const double fx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)];
+      const double fy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N];
+      const double fz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P];
+      const double grad = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / k.a2;
+      const double kin = 0.5 * (p * p);
+      const double q2 = (f * f) * (f * f);
+      const double pot = k.cm * (f * f) + k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
+      two_acc(hi[0], lo[0], f);
+      two_acc(hi[1], lo[1], pot);
+      two_acc(hi[2], lo[2], kin);
+      two_acc(hi[3], lo[3], grad);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const double h2 = __shfl_down(hi[j], off, 64), l2 = __shfl_down(lo[j], off, 64);
+        two_merge(hi[j], lo[j], h2, l2);
+      }
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < 4; j++) idbl[(long long)id * kND + j] = hi[j] + lo[j];
   }
 }
 
@@ -376,9 +484,12 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
   hipLaunchKernelGGL(k_isl_scan, dim3(1), dim3(1024), 0, c->stream, s->blk, s->nblk);
   hipLaunchKernelGGL(k_isl_assign, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3,
                      s->blk, s->cid);
+  hipLaunchKernelGGL(k_isl_mcount, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3, s->mblk);
+  hipLaunchKernelGGL(k_isl_scan, dim3(1), dim3(1024), 0, c->stream, s->mblk, s->nblk);
   PN_CHECK_LAUNCH(c);
-  int nisl = 0;
+  int nisl = 0, nsite = 0;
   PN_HIP(c, hipMemcpyAsync(&nisl, s->blk + s->nblk, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipMemcpyAsync(&nsite, s->mblk + s->nblk, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   PN_HIP(c, hipStreamSynchronize(c->stream));
   out->count = nisl;
   if (nisl > s->cap) {
@@ -386,12 +497,42 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
     hipFree(s->iint);
     hipFree(s->idbl);
     s->ikey = nullptr; s->iint = nullptr; s->idbl = nullptr;
+    hipFree(s->ist);
+    s->ist = nullptr;
     s->cap = 0;
     const int cap = std::max(nisl, 1024);
     PN_HIP(c, hipMalloc(&s->ikey, (size_t)cap * sizeof(unsigned long long)));
     PN_HIP(c, hipMalloc(&s->iint, (size_t)cap * kNI * sizeof(long long)));
     PN_HIP(c, hipMalloc(&s->idbl, (size_t)cap * kND * sizeof(double)));
+    PN_HIP(c, hipMalloc(&s->ist, ((size_t)cap + 1) * sizeof(int)));
     s->cap = cap;
+  }
+  // island sites grouped by island: compaction in raster order, stable sort
+  // by island number (radix sort over the bits island numbers use)
+  int end_bit = 1;
+  while (end_bit < 31 && (1LL << end_bit) <= (long long)nisl) end_bit++;
+  if (nisl > 0) {
+    if (nsite > s->mcap) {
+      hipFree(s->mkey); hipFree(s->mkey2); hipFree(s->msite); hipFree(s->msite2);
+      s->mkey = s->mkey2 = s->msite = s->msite2 = nullptr;
+      s->mcap = 0;
+      const size_t m = (size_t)nsite;
+      PN_HIP(c, hipMalloc(&s->mkey, m * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->mkey2, m * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->msite, m * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->msite2, m * sizeof(int)));
+      s->mcap = nsite;
+    }
+    size_t bytes = 0;
+    PN_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, s->mkey, s->mkey2, s->msite, s->msite2,
+                                                 nsite, 0, end_bit, c->stream));
+    if (bytes > s->sort_bytes) {
+      hipFree(s->sort_tmp);
+      s->sort_tmp = nullptr;
+      s->sort_bytes = 0;
+      PN_HIP(c, hipMalloc(&s->sort_tmp, bytes));
+      s->sort_bytes = bytes;
+    }
   }
   if (nisl > 0) {
     hipLaunchKernelGGL(k_isl_prep, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, nisl,
@@ -399,6 +540,15 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
     hipLaunchKernelGGL(k_isl_roots, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, s->iint);
     hipLaunchKernelGGL(k_isl_stats<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
                        s->cid, s->ikey, s->iint, s->idbl);
+    hipLaunchKernelGGL(k_isl_mscatter, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3,
+                       s->mblk, s->cid, s->mkey, s->msite);
+    size_t bytes = s->sort_bytes;
+    PN_HIP(c, hipcub::DeviceRadixSort::SortPairs(s->sort_tmp, bytes, s->mkey, s->mkey2, s->msite,
+                                                 s->msite2, nsite, 0, end_bit, c->stream));
+    hipLaunchKernelGGL(k_isl_starts, dim3((nsite + 255) / 256), dim3(256), 0, c->stream, s->mkey2, nsite,
+                       nisl, s->ist);
+    hipLaunchKernelGGL(k_isl_sums<T>, dim3((unsigned)std::min(4096, (nisl + 3) / 4)), dim3(kT), 0,
+                       c->stream, phi, pi, N, k, s->ist, s->msite2, nisl, s->idbl);
     hipLaunchKernelGGL(k_isl_peak<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
                        s->cid, s->ikey, s->iint);
     hipLaunchKernelGGL(k_isl_peakval<T>, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, phi, pi,
@@ -472,6 +622,7 @@ extern "C" int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out) {
     PN_HIP(c, hipMalloc(&s->label, (size_t)c->n3 * sizeof(int)));
     PN_HIP(c, hipMalloc(&s->cid, (size_t)c->n3 * sizeof(int)));
     PN_HIP(c, hipMalloc(&s->blk, (size_t)(s->nblk + 1) * sizeof(int)));
+    PN_HIP(c, hipMalloc(&s->mblk, (size_t)(s->nblk + 1) * sizeof(int)));
   }
   IslCoef k;
   const int Nc = 1;

@@ -92,3 +92,43 @@ def test_spectrum_sums_match_exact_sums(N, prec):
             exact = math.fsum(terms)
             tol = 4 * np.spacing(abs(exact)) if key != "Q" else 4 * np.spacing(math.fsum(np.abs(terms)))
             assert abs(s1[key][k] - exact) <= tol, (key, k, s1[key][k], exact)
+
+
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_island_sums_match_exact_sums(prec):
+    # per-island phi, potential, kinetic and gradient sums (islands.cpp:131-142)
+    # against math.fsum over the island's sites; the same bits on a second call
+    N, a = 24, 0.5
+    m = O.model(N)
+    rng = np.random.default_rng(17)
+    phi = 2.4 + 0.8 * rng.standard_normal(N ** 3)
+    pi = 0.5 * rng.standard_normal(N ** 3)
+    thr = 3.0
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"], precision=prec) as L:
+        L.set_fields(phi, pi)
+        f, p = L.get_fields()
+        rows1, lab, _ = L.islands(0, thr, labels=True)
+        rows2, _, _ = L.islands(0, thr)
+    assert len(rows1) > 20
+    fields = ("avg_phi", "pot_en", "kin_en", "grad_en", "peak_value", "cx", "cy", "cz")
+    assert [[getattr(r, k) for k in fields] for r in rows1] == [[getattr(r, k) for k in fields] for r in rows2]
+    F = f.reshape(N, N, N)
+    fx = np.roll(F, -1, axis=2).ravel()
+    fy = np.roll(F, -1, axis=1).ravel()
+    fz = np.roll(F, -1, axis=0).ravel()
+    a2 = a * a
+    grad = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / a2
+    kin = 0.5 * (p * p)
+    q2 = (f * f) * (f * f)
+    cm = m["mass_mode"] * 0.5 * m["m2"]
+    d6 = 720.0 * m["kappa"] * m["kappa"]
+    pot = cm * (f * f) + m["lambda_mode"] * (q2 / 24.0) + (q2 * (f * f) / d6)
+    a3 = (a * a) * a
+    for r in rows1:
+        sel = lab == r.index
+        nn = int(sel.sum())
+        assert nn == r.nn
+        for got, terms, post in ((r.avg_phi, f[sel], lambda s: s / nn), (r.pot_en, pot[sel], lambda s: s * a3),
+                                 (r.kin_en, kin[sel], lambda s: s * a3), (r.grad_en, grad[sel], lambda s: s * a3)):
+            want = post(math.fsum(terms))
+            assert abs(got - want) <= 4 * np.spacing(math.fsum(np.abs(post(terms)))), (r.index, got, want)
