@@ -19,6 +19,7 @@ import argparse
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -320,22 +321,7 @@ def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, check_steps=1000, 
     return out
 
 
-def cpu_baseline(N, m, target_s=12.0):
-    import ctypes as C
-    path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
-    if not os.path.exists(path):
-        return None
-    lib = C.CDLL(path)
-    lib.cb_run.restype = C.c_double
-    lib.cb_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
-                           C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    ck = C.c_double()
-    probe = lib.cb_run(N, threads, 1, 3, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
-                       m["kappa"], 0.1, m["phi0"], C.byref(ck))
-    steps = max(3, int(target_s / max(probe / 3.0, 1e-6)))
-    sec = lib.cb_run(N, threads, 2, steps, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
-                     m["kappa"], 0.1, m["phi0"], C.byref(ck))
+def _host_info(threads):
     model_name = "unknown CPU"
     try:
         for ln in open("/proc/cpuinfo"):
@@ -344,13 +330,79 @@ def cpu_baseline(N, m, target_s=12.0):
                 break
     except OSError:
         pass
+    return {"cpu": model_name, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def _host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, n))
+
+
+def cpu_baseline_reference(N, m, target_s=12.0):
+    """The reference's own evolve() (oracle/_ref/ref_timing, built from the
+    reference sources with its flags) over MPI z-slabs on the host's cores,
+    one rank per core: a probe of a few steps sizes a run of about target_s.
+    None when the build or mpirun is missing or the run fails."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_timing")
+    mpirun = "/opt/conda/bin/mpirun"
+    if not (os.path.exists(exe) and os.path.exists(mpirun)):
+        return None
+    ranks = _host_threads()
+    while ranks > 1 and N % ranks:
+        ranks -= 1
+    args = ["-Nt", str(N), "-at", repr(m["a"]), "-M", repr(m["m2"]), "-Mmode", str(m["mass_mode"]),
+            "-L", repr(m["Lambda"]), "-Lmode", str(m["lambda_mode"]), "-Kappa", repr(m["kappa"]),
+            "-T0", "0", "-dt", "0.1", "-phi0_0", repr(m["phi0"])]
+
+    def run(steps):
+        r = subprocess.run([mpirun, "-np", str(ranks), exe, str(steps)] + args, capture_output=True,
+                           text=True, timeout=300, cwd="/tmp")
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-400:])
+        return float(r.stdout.strip().splitlines()[-1])
+
+    try:
+        probe = run(4)
+        steps = max(4, int(target_s / max(probe / 4.0, 1e-6)))
+        sec = run(steps)
+    except (RuntimeError, ValueError, OSError, subprocess.TimeoutExpired) as e:
+        log("reference CPU baseline unavailable: %s" % e)
+        return None
+    return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": ranks,
+            "kind": "reference",
+            "sample": "%d^3 FP64 leapfrog, %d timed steps (%.1f s) of the reference's own evolve() "
+                      "(dynamics.cpp:196-237, halo exchange comm.cpp:63-78) built from its sources with "
+                      "its flags -O3 -msse4.2 -fprefetch-loop-arrays -funroll-all-loops -ffast-math, "
+                      "mpirun -np %d z-slabs" % (N, steps, sec, ranks),
+            "host": _host_info(ranks)}
+
+
+def cpu_baseline(N, m, target_s=12.0):
+    ref = cpu_baseline_reference(N, m, target_s)
+    if ref is not None:
+        return ref
+    import ctypes as C
+    path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.cb_run.restype = C.c_double
+    lib.cb_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                           C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    threads = _host_threads()
+    ck = C.c_double()
+    probe = lib.cb_run(N, threads, 1, 3, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
+                       m["kappa"], 0.1, m["phi0"], C.byref(ck))
+    steps = max(3, int(target_s / max(probe / 3.0, 1e-6)))
+    sec = lib.cb_run(N, threads, 2, steps, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
+                     m["kappa"], 0.1, m["phi0"], C.byref(ck))
     return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": threads,
             "kind": "port",
             "sample": "%d^3 FP64 leapfrog, %d timed steps (%.1f s) of the reference loop nest "
                       "(AoS + z halo, pos()/mod() per site, kick then drift, z slabs over %d threads), "
                       "reference flags -O3 -ffast-math" % (N, steps, sec, threads),
-            "host": {"cpu": model_name, "nproc": os.cpu_count(),
-                     "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}}
+            "host": _host_info(threads)}
 
 
 def main():
