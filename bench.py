@@ -504,7 +504,7 @@ def main():
         }
         N5 = 512
         log("fp64 %d^3" % N5)
-        ms5, _, on5 = run_leg(pn, N5, pn.PN_FP64, 200, 20, 200, 5, m)
+        ms5, _, on5 = run_leg(pn, N5, pn.PN_FP64, 200, 40, 200, 5, m)
         line["fp64_512"] = {"ms_per_step": ms5 / 200,
                             "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
                             "roofline": roofline("k_step2<double, CoefF64, true, 4, 6, true, 3>", N5, ms5, 200,

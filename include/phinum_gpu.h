@@ -184,6 +184,12 @@ int pn_sync(pn_ctx *c);
  * on the device (pn_ic_modes_device); 0 draws them on the host (pn_ic_modes)
  * and uploads them. */
 #define PN_OPT_IC_ON_DEVICE 2
+/* PN_OPT_STEP2_ZBLOCK: planes per tile of the two-step kernel.  Reads the z
+ * block in use (the shape's default until the first pn_evolve call with at
+ * least 32 steps has timed its first pairs of steps with z blocks 8 ... 64
+ * and kept the fastest; all give the same bits).  Setting a value > 0 fixes
+ * it; 0 returns to the default and tunes again. */
+#define PN_OPT_STEP2_ZBLOCK 3
 int pn_set_option(pn_ctx *c, int option, int value);
 int pn_get_option(pn_ctx *c, int option, int *value);
 /* pn_evolve bracketed by HIP events on the context's stream; waits for the

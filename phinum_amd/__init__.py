@@ -24,6 +24,7 @@ PN_FP64, PN_FP32 = 0, 1
 PN_LAYOUT_REF, PN_LAYOUT_DENSE = 0, 1
 PN_OPT_TWO_STEP = 1
 PN_OPT_IC_ON_DEVICE = 2
+PN_OPT_STEP2_ZBLOCK = 3
 
 
 class PnParams(C.Structure):
@@ -347,6 +348,15 @@ class Lattice:
     def initial_conditions(self, ic):
         """SetInitialConditions (initialize==3) with a PnIcParams."""
         self._chk(self.L.pn_initial_conditions(self.h, C.byref(ic)))
+
+    def step2_zblock(self, zb=None):
+        """Gets or sets PN_OPT_STEP2_ZBLOCK (planes per two-step tile; 0 tunes
+        it again on the next evolve call of at least 32 steps)."""
+        if zb is not None:
+            self._chk(self.L.pn_set_option(self.h, PN_OPT_STEP2_ZBLOCK, int(zb)))
+        v = C.c_int()
+        self._chk(self.L.pn_get_option(self.h, PN_OPT_STEP2_ZBLOCK, C.byref(v)))
+        return v.value
 
     def ic_on_device(self, on=None):
         """Gets or sets PN_OPT_IC_ON_DEVICE (modes drawn on the device, the

@@ -632,6 +632,57 @@ extern "C" int pn_drift(pn_ctx *c, double dtd) {
   return PN_OK;
 }
 
+// Online choice of the two-step z block.  The first evolve call with at least
+// kTuneSteps steps advances its first pairs of steps with each candidate z
+// block in turn (every z block gives the same bits), times them with events,
+// and keeps the fastest: the number of tiles against the device's CUs, the
+// redundant halo planes and the occupancy are traded off by measurement.  A
+// candidate must beat the shape's default by 3 %.
+static constexpr int kTuneSteps = 32;
+
+static int tune_step2_zb(pn_ctx *c, Step2Shape *s2, const double *dtk, const double *dtd, int *n) {
+  static const int cand[] = {8, 12, 16, 24, 32, 48, 64};
+  hipEvent_t e0, e1;
+  PN_HIP(c, hipEventCreate(&e0));
+  PN_HIP(c, hipEventCreate(&e1));
+  auto pair = [&](int zb, float *ms) -> int {
+    Step2Shape t = *s2;
+    t.ZB = std::min(zb, c->p.N);
+    hipError_t e = hipEventRecord(e0, c->stream);
+    int rc = e == hipSuccess ? launch_step2(c, t, dtk + *n, dtd + *n) : PN_ERR_HIP;
+    *n += 2;
+    if (rc == PN_OK && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (rc == PN_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+    if (rc == PN_OK && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (rc == PN_OK && e != hipSuccess) rc = pn_fail(c, PN_ERR_HIP, "pn_evolve: %s", hipGetErrorString(e));
+    return rc;
+  };
+  const int def = s2->ZB;
+  float ms = 0.f, best_ms = 0.f;
+  int rc = pair(def, &ms);  // warms the caches
+  int best = def;
+  for (int i = 0; rc == PN_OK && i < (int)(sizeof(cand) / sizeof(cand[0])); i++) {
+    if (cand[i] > c->p.N) continue;
+    float t = 1e30f;
+    for (int r = 0; rc == PN_OK && r < 2; r++) {
+      rc = pair(cand[i], &ms);
+      t = std::min(t, ms);
+    }
+    if (cand[i] == def) t *= 0.97f;  // the default wins near-ties
+    if (rc == PN_OK && (best_ms == 0.f || t < best_ms)) {
+      best_ms = t;
+      best = cand[i];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PN_OK) return rc;
+  c->step2_zb = best;
+  c->step2_tuned = 1;
+  s2->ZB = std::min(best, c->p.N);
+  return PN_OK;
+}
+
 extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double *dtd) {
   if (!c) return pn_fail(nullptr, PN_ERR_ARG, "pn_evolve: NULL context");
   if (nsteps < 0 || (nsteps > 0 && (!dtk || !dtd)))
@@ -641,6 +692,11 @@ extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double 
   int n = 0;
   Step2Shape s2;
   if (c->step2 && c->pi_alt && step2_shape(c, &s2)) {
+    if (c->step2_zb > 0) s2.ZB = std::min(c->step2_zb, c->p.N);
+    else if (!c->step2_tuned && nsteps >= kTuneSteps) {
+      int rc = tune_step2_zb(c, &s2, dtk, dtd, &n);
+      if (rc != PN_OK) return rc;
+    }
     for (; n + 1 < nsteps; n += 2) {
       int rc = launch_step2(c, s2, dtk + n, dtd + n);
       if (rc != PN_OK) return rc;
@@ -664,6 +720,12 @@ extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
     c->ic_device = value ? 1 : 0;
     return PN_OK;
   }
+  if (option == PN_OPT_STEP2_ZBLOCK) {
+    if (value < 0) return pn_fail(c, PN_ERR_ARG, "pn_set_option: z block %d", value);
+    c->step2_zb = value;        // 0: the default shape, tuned again by the next evolve call
+    c->step2_tuned = value > 0;
+    return PN_OK;
+  }
   return pn_fail(c, PN_ERR_ARG, "pn_set_option: unknown option %d", option);
 }
 
@@ -675,6 +737,11 @@ extern "C" int pn_get_option(pn_ctx *c, int option, int *value) {
   }
   if (option == PN_OPT_IC_ON_DEVICE) {
     *value = c->ic_device;
+    return PN_OK;
+  }
+  if (option == PN_OPT_STEP2_ZBLOCK) {
+    Step2Shape s2;
+    *value = c->step2_zb > 0 ? c->step2_zb : (step2_shape(c, &s2) ? s2.ZB : 0);
     return PN_OK;
   }
   return pn_fail(c, PN_ERR_ARG, "pn_get_option: unknown option %d", option);

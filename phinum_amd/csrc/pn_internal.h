@@ -34,6 +34,8 @@ struct pn_ctx {
   int step2;           // pn_evolve advances pairs of steps in one pass
   int lds_max;         // LDS bytes a block may use on this device
   int ic_device = 1;   // initial-condition modes drawn on the device (else on the host)
+  int step2_zb = 0;    // two-step z block: 0 = the shape's default until tuned
+  int step2_tuned = 0; // z block chosen by timing (or fixed by the caller)
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch

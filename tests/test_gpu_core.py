@@ -85,6 +85,35 @@ def test_two_step_kernel_bitwise_equals_single_steps_large(N, steps, prec):
     assert d["sum_phi2"] > 0 and d["max_pi"] > 0
 
 
+@pytest.mark.parametrize("N,prec", [(40, pn.PN_FP64), (40, pn.PN_FP32), (130, pn.PN_FP64)])
+def test_two_step_zblocks_and_tuning_bitwise(N, prec):
+    # every z block gives the bits of single steps, and so does the first
+    # long evolve call, which times its own pairs of steps with each z block
+    m = O.model(N)
+    phi, pi = _fields(N, 90 + N)
+    steps = 40
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    with _lat(m, prec) as L:
+        L.two_step(False)
+        L.set_fields(phi, pi)
+        L.evolve(dtk, dtd)
+        ref = L.get_fields()
+    runs = [8, 12, 24, N, 0]  # 0: tuned by the evolve call
+    for zb in runs:
+        with _lat(m, prec) as L:
+            assert L.two_step(True)
+            L.step2_zblock(zb)
+            L.set_fields(phi, pi)
+            L.evolve(dtk, dtd)
+            got = L.get_fields()
+            used = L.step2_zblock()
+        assert np.array_equal(_bits(got[0]), _bits(ref[0])) and np.array_equal(_bits(got[1]), _bits(ref[1])), zb
+        if zb:
+            assert used == zb
+        else:
+            assert used in (8, 12, 16, 24, 32, 48, 64)
+
+
 def test_two_step_unavailable_is_reported():
     m = O.model(33)  # odd N: no vector width fits
     with _lat(m) as L:
