@@ -47,6 +47,23 @@ static void check(int rc, const char *what) {
   }
 }
 
+// Steps taken by evolve() on the host clock (tau, Last_*_update and the step
+// counts advance at once, as in the reference) but not yet launched.  They
+// go to the device in one pn_evolve call, so pairs of steps share a pass over
+// HBM: before anything reads or writes the device fields, and every
+// kMaxPending steps.  The fields are the same bits as with one launch per step.
+static std::vector<double> g_pend_dtk, g_pend_dtd;
+static constexpr size_t kMaxPending = 256;
+
+static void flush_steps() {
+  if (g_pend_dtk.empty()) return;
+  const int n = (int)g_pend_dtk.size();
+  const int rc = pn_evolve(LocalLattice::ctx, n, g_pend_dtk.data(), g_pend_dtd.data());
+  g_pend_dtk.clear();
+  g_pend_dtd.clear();
+  check(rc, "evolve");
+}
+
 // -key value pairs as Konfig (cfile.c:55-73): a later duplicate overwrites,
 // values are read with atof/atoi (cfile.c:252-278), unknown keys are ignored.
 void GetParameters(int argc, char **argv) {
@@ -140,6 +157,8 @@ void init() {  // lattice.cpp:70-104
 }
 
 void fin() {
+  g_pend_dtk.clear();  // nothing can observe them any more
+  g_pend_dtd.clear();
   delete[] Phi;
   delete[] Pi;
   delete[] phi;
@@ -151,8 +170,14 @@ void fin() {
   ctx = nullptr;
 }
 
-void SyncToHost() { check(pn_get_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToHost"); }
-void SyncToDevice() { check(pn_set_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToDevice"); }
+void SyncToHost() {
+  flush_steps();
+  check(pn_get_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToHost");
+}
+void SyncToDevice() {
+  flush_steps();
+  check(pn_set_fields(ctx, Phi, Pi, PN_LAYOUT_REF), "SyncToDevice");
+}
 }  // namespace LocalLattice
 
 // ---- renormalization.cpp / initialconditions.cpp ---------------------------
@@ -161,6 +186,7 @@ void fix_tadpole() {  // renormalization.cpp:3-31 with renorm_tadpole == 0
   LocalLattice::BareM_sqr_trans = M_sqr;
   M2_long = M_sqr;
   M2_trans = M_sqr;
+  flush_steps();  // steps already taken used the old couplings
   check(pn_set_couplings(LocalLattice::ctx, Mass_Mode, LocalLattice::BareM_sqr_long, Lambda_Mode, Kappa),
         "fix_tadpole");
 }
@@ -176,6 +202,7 @@ void SetInitialConditions() {
   ic.justnoise = justnoise;
   ic.phi0 = phi0_0;
   ic.pi0 = pi0_0;
+  flush_steps();
   check(pn_initial_conditions(LocalLattice::ctx, &ic), "SetInitialConditions");
 }
 
@@ -196,16 +223,18 @@ void reset() {  // dynamics.cpp:28-50
 }
 
 void updateMomenta() {  // dynamics.cpp:52-175
+  flush_steps();
   check(pn_kick(LocalLattice::ctx, LocalLattice::tau - numeric::Last_Momenta_update), "updateMomenta");
   numeric::Last_Momenta_update = LocalLattice::tau;
 }
 
 void updateFields() {  // dynamics.cpp:177-194
+  flush_steps();
   check(pn_drift(LocalLattice::ctx, LocalLattice::tau - numeric::Last_Field_update), "updateFields");
   numeric::Last_Field_update = LocalLattice::tau;
 }
 
-void evolve() {  // dynamics.cpp:196-237: kick and drift fused into one launch
+void evolve() {  // dynamics.cpp:196-237: kick and drift fused, launched by flush_steps
   using LocalLattice::tau;
   tau += 0.5 * LocalLattice::dtau;
   const double dtk = tau - numeric::Last_Momenta_update;
@@ -213,7 +242,9 @@ void evolve() {  // dynamics.cpp:196-237: kick and drift fused into one launch
   tau += 0.5 * LocalLattice::dtau;
   const double dtd = tau - numeric::Last_Field_update;
   numeric::Last_Field_update = tau;
-  check(pn_evolve(LocalLattice::ctx, 1, &dtk, &dtd), "evolve");
+  g_pend_dtk.push_back(dtk);
+  g_pend_dtd.push_back(dtd);
+  if (g_pend_dtk.size() >= kMaxPending) flush_steps();
   numeric::normal_stepcount++;
   Communication::exchange();
   numeric::total_stepcount++;
@@ -229,6 +260,7 @@ void exchange() {}
 void CalculateVolumeAverages() {
   using namespace LocalLattice;
   pn_averages a;
+  flush_steps();
   check(pn_volume_averages(ctx, &a), "CalculateVolumeAverages");
   phi[0] = avgphi[0] = a.phi;
   pi[0] = avgpi[0] = a.pi;
@@ -245,6 +277,7 @@ void CalculateVolumeAverages() {
 void CalculateExtrema() {
   using namespace LocalLattice;
   pn_extrema e;
+  flush_steps();
   check(pn_extrema_get(ctx, &e), "CalculateExtrema");
   GlminPhi = minPhi = e.min;
   GlmaxPhi = maxPhi = e.max;
@@ -346,6 +379,7 @@ void Calculate_CorrelationFunctions() {  // momentumspace.cpp:140-278
   s.corr_f = TotalCorrF.data();
   s.corr_k = TotalCorrK.data();
   s.corr_q = TotalCorrQ.data();
+  flush_steps();
   check(pn_spectrum_get(LocalLattice::ctx, &s), "Calculate_CorrelationFunctions");
 }
 
@@ -417,6 +451,7 @@ void Full_Analysis() {
   char dir[64];
   std::snprintf(dir, sizeof dir, "./output/%i", LASTRUN);
   const bool files = write_output && ID == 0;
+  flush_steps();
   int rc = pn_tracker_update(tracker, LocalLattice::ctx, island_mode, thres, LocalLattice::tau,
                              files ? dir : nullptr);
   if (rc != PN_OK) throw std::runtime_error(std::string("Full_Analysis: ") + pn_tracker_last_error(tracker));
