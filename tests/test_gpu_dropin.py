@@ -125,3 +125,29 @@ def test_driver_checkpoint_round_trip(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert (tmp_path / "field-8-0.out").read_text() == field
     assert (tmp_path / "moment-8.out").read_text() == (tmp_path / "moment-7.out").read_text()
+
+
+def test_driver_queued_steps_bitwise(tmp_path):
+    # evolve() queues its device work; 300 steps without an output in between
+    # pass the 256-step flush inside the run.  Resumed from exact fields
+    # (-extractconfig) and saved at the end (-saveconfig), the fields are the
+    # oracle's bit for bit.
+    N, TMax = 16, 15.0
+    m = O.model(N)
+    rng = np.random.default_rng(5)
+    phi = np.sqrt(6.0) + 0.4 * rng.standard_normal(N ** 3)
+    pi = 0.2 * rng.standard_normal(N ** 3)
+    (tmp_path / "field-5-0.out").write_text(O.config_text(phi, pi))
+    (tmp_path / "moment-5.out").write_text(O.moment_text(0.0))
+    args = [MAIN, "-runs", "1", "-Nt", str(N), "-at", "0.5", "-M", "1", "-Mmode", "1", "-L", "6",
+            "-Lmode", "-1", "-Kappa", "%.17g" % m["kappa"], "-T0", "0", "-TMax", "%g" % TMax,
+            "-dt", "0.1", "-TSO", "1000", "-TMO", "1000", "-Noise", "0.5", "-Ampl", "0",
+            "-phi0_0", "%.17g" % np.sqrt(6.0), "-seed", "1", "-output", "0",
+            "-extractconfig", "5", "-saveconfig", "9"]
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, taus = O.time_factors(400)
+    steps = int(np.argmax(taus >= TMax)) + 1
+    assert steps > 256
+    ophi, opi = O.evolve(m, phi, pi, steps)
+    assert (tmp_path / "field-9-0.out").read_text() == O.config_text(ophi, opi)
