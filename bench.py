@@ -4,16 +4,18 @@ Workload: the 256^3 FP64 drop-in configuration (dx = 0.5, dt = 0.05,
 V = phi^2/2 - phi^4/4 + 0.2 phi^6/6 in the reference's units, 10^4 steps with
 energy outputs every 200 steps).  `value` is FP64 lattice site-updates per
 second with the fields resident in HBM, timed with HIP events on the stream
-the step kernels run on.  The steps run as the two-step (temporally blocked)
-kernel, bit-identical to single steps; `roofline` reports that kernel against
-its own algorithmic traffic (Phi and Pi read and written once per launch of
-two steps) and, beside it, the site-update rate against the one-step bound
-(32 B per site-update).  The line also carries the single-step kernel, the
+the step kernels run on.  The steps run as the multi-step (temporally
+blocked) kernel the library picks by timing during the warmup (two or three
+steps per pass over HBM, bit-identical to single steps); `roofline` reports
+that kernel against its own algorithmic traffic (Phi and Pi read and written
+once per launch) and, beside it, the site-update rate against the one-step
+bound (32 B per site-update).  The line also carries the single-step kernel, the
 FP32 form (FP32 fields, FP64 compensated energy sums) with its energy
 difference to the FP64 run, the 512^3 case, the batched ensemble, the
 adjoint, and the CPU baseline.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-extra]
+                [--kernel S:Z  (fix S steps per pass with z block Z instead of the timed choice)]
 """
 import argparse
 import json
@@ -22,6 +24,8 @@ import re
 import subprocess
 import sys
 import time
+
+import ctypes as C
 
 import numpy as np
 
@@ -45,24 +49,35 @@ def model(lam=1.0, g=0.2, phi0=1.0):
                 phi0=float(np.sqrt(L)) * phi0, Lambda=L)
 
 
-def launches(steps, chunk, two_step):
-    """(two-step launches, single-step launches) for `steps` run in chunks."""
+def launches(steps, chunk, per_pass):
+    """Launch counts for `steps` run in chunks of `chunk` with the kernel
+    advancing `per_pass` (1, 2 or 3) steps per launch: (main launches, steps
+    left to shorter launches).  pn_evolve runs the rest of a chunk by pairs,
+    then singly."""
     full, rest = divmod(steps, chunk)
     sizes = [chunk] * full + ([rest] if rest else [])
-    if not two_step:
-        return 0, steps
-    return sum(n // 2 for n in sizes), sum(n % 2 for n in sizes)
+    return sum(n // per_pass for n in sizes), sum(n % per_pass for n in sizes)
 
 
-def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True):
+def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True, kernel=None):
     """Seeded initialize==3 IC on the device, then `steps` timed steps in chunks
     of `out_every` with an FP64 energy between chunks (outside the timing).
-    Returns (total event ms over `steps`, energies at each output, two-step on)."""
+    The multi-step kernel is the library's own choice (timed during a warmup of
+    at least 64 steps) unless `kernel` = (steps per pass, z block) fixes it.
+    Returns (total event ms over `steps`, energies at each output, the kernel
+    used as {"steps_per_pass", "zblock"})."""
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)
     energies = []
     with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
                     precision=precision) as L:
-        on = L.two_step() if two_step else L.two_step(False)  # on by default where a shape fits
+        if not two_step:
+            L.two_step(False)
+        elif kernel:
+            if kernel[0] == 3:
+                L.three_step(True, zblock=kernel[1])
+            else:
+                L.three_step(False)
+                L.step2_zblock(kernel[1])
         L.initial_conditions(pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
         if warmup:
             L.evolve(dtk[:warmup], dtd[:warmup])
@@ -74,7 +89,15 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True):
             total += L.evolve_timed(dtk[s:s + n], dtd[s:s + n])
             s += n
             energies.append(L.volume_averages().energy)
-    return total, np.array(energies), on
+        if not L.two_step():
+            used = {"steps_per_pass": 1, "zblock": None}
+        elif L.three_step():
+            zb = C.c_int()
+            L._chk(L.L.pn_get_option(L.h, pn.PN_OPT_STEP3_ZBLOCK, C.byref(zb)))
+            used = {"steps_per_pass": 3, "zblock": zb.value}
+        else:
+            used = {"steps_per_pass": 2, "zblock": L.step2_zblock()}
+    return total, np.array(energies), used
 
 
 def profiled_value(name, key):
@@ -95,20 +118,30 @@ def profiled_traffic(name):
     return int(t) if t else None
 
 
-def roofline(kernel, N, ms_total, steps, chunk, two_step, esize, profile=None):
+def kernel_names(t, per_pass, N):
+    """(kernel name for the JSON line, profile summary name) of the kernel that
+    advances `per_pass` steps per launch for element type t."""
+    p = "fp64" if t == "double" else "fp32"
+    c = "CoefF64" if t == "double" else "CoefF32"
+    if per_pass == 3:
+        return ("k_step3<%s, %s, POW2, W, RT, FAST> (three leapfrog steps per pass)" % (t, c),
+                "k_step3_%s_%dcube" % (p, N))
+    if per_pass == 2:
+        return ("k_step2<%s, %s, POW2, W, RT, LP, FAST> (two leapfrog steps per pass)" % (t, c),
+                "k_step2_%s_%dcube" % (p, N))
+    return "k_step_vec<%s, %s, POW2, W, 4, ZB, 0> (one fused step)" % (t, c), "k_step_vec_%s_%dcube" % (p, N)
+
+
+def roofline(kernel, N, ms_total, steps, chunk, per_pass, esize, profile=None):
     """Roofline of the dominant kernel: algorithmic bytes per launch (Phi and
     Pi read once and written once: 4 * esize bytes per site) over the average
     launch duration, plus the site-update rate against the one-step bound."""
-    n2, n1 = launches(steps, chunk, two_step)
+    nmain, nrest = launches(steps, chunk, per_pass)
     sites = float(N) ** 3
     algo = 4.0 * esize * sites
-    if n2:
-        # single-step launches in odd chunks take about half a two-step launch
-        per_launch_ms = ms_total / (n2 + 0.5 * n1)
-        steps_per_launch = 2
-    else:
-        per_launch_ms = ms_total / n1
-        steps_per_launch = 1
+    # the few steps of a chunk left to shorter launches count pro rata
+    per_launch_ms = ms_total / (nmain + nrest / float(per_pass))
+    steps_per_launch = per_pass
     achieved = algo / (per_launch_ms * 1e-3) / 1e9
     upd = sites * steps / (ms_total * 1e-3)
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -191,7 +224,8 @@ def precision_leg(pn, N, steps, out_every, seed=1234):
             Lc.evolve(dtk[s:s + n], dtd[s:s + n])
             s += n
             record(s)
-        on64, on32 = L64.two_step(), L32.two_step()
+        on64, on32 = ("three-step" if L.three_step() else "two-step" if L.two_step() else "one-step"
+                      for L in (L64, L32))
     dE = np.array([r["dE"] for r in rows])
     over = np.nonzero(dE > 1e-6)[0]
     e64 = np.array([r["e64"] for r in rows])
@@ -199,7 +233,7 @@ def precision_leg(pn, N, steps, out_every, seed=1234):
     v64 = float(N) ** 3 * steps / (ms["fp64"] * 1e-3)
     v32 = float(N) ** 3 * steps / (ms["fp32"] * 1e-3)
     return {"N": N, "steps": steps, "out_every": out_every, "lambda": 1.2, "g": 0.25, "phi0": 1.2,
-            "seed": seed, "fp64_two_step_kernel": on64, "fp32_two_step_kernel": on32,
+            "seed": seed, "fp64_kernel": on64, "fp32_kernel": on32,
             "fp64_ms_per_step": ms["fp64"] / steps, "fp32_ms_per_step": ms["fp32"] / steps,
             "fp64_site_updates_per_s": v64, "fp32_site_updates_per_s": v32, "ratio_fp32_vs_fp64": v32 / v64,
             "steps_with_dE_over_E_le_1e-6": int(steps if over.size == 0 else rows[over[0] - 1]["step"]),
@@ -409,7 +443,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--out-every", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=5000)
@@ -421,6 +455,9 @@ def main():
     ap.add_argument("--precision-only", action="store_true",
                     help="only the precision form (FP32 against FP64 at --precision-size)")
     ap.add_argument("--single-step", action="store_true", help="time the one-step kernel only")
+    ap.add_argument("--kernel", default=None,
+                    help="S:Z fixes the multi-step kernel (S = 2 or 3 steps per pass, z block Z) "
+                         "instead of the library's timed choice (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -450,20 +487,19 @@ def main():
         return
     N, K, W = args.size, args.steps, args.warmup
     two = not args.single_step
+    kern = tuple(int(v) for v in args.kernel.split(":")) if args.kernel else None
     m = model()
     if args.fp32_only:
-        ms32, _, on32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
-        print(json.dumps({"fp32_ms_per_step": ms32 / K, "two_step_kernel": on32}), flush=True)
+        ms32, _, k32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m, kernel=kern)
+        print(json.dumps({"fp32_ms_per_step": ms32 / K, "kernel": k32}), flush=True)
         return
-    log("fp64 %d^3: %d warmup + %d timed steps (two-step kernel %s)" % (N, W, K, "on" if two else "off"))
+    log("fp64 %d^3: %d warmup + %d timed steps" % (N, W, K))
     t0 = time.time()
-    ms64, e64, on = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m, two_step=two)
-    log("fp64 done in %.1fs wall, %.4f ms/step" % (time.time() - t0, ms64 / K))
+    ms64, e64, k64 = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m, two_step=two, kernel=kern)
+    log("fp64 done in %.1fs wall, %.4f ms/step, kernel %s" % (time.time() - t0, ms64 / K, k64))
     sites = float(N) ** 3
     value = sites * K / (ms64 * 1e-3)
-    kname = ("k_step2<double, CoefF64, POW2, W, RT, LP, FAST> (two leapfrog steps per pass)" if on else
-             "k_step_vec<double, CoefF64, POW2, 2, 4, ZB, 0> (one fused step)")
-    prof = ("k_step2_fp64_%dcube" if on else "k_step_vec_fp64_%dcube") % N
+    kname, prof = kernel_names("double", k64["steps_per_pass"], N)
     line = {
         "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": ms64 / K, "higher_is_better": True, "scaling": "weak",
@@ -472,8 +508,8 @@ def main():
                 "modes below k_max/2), generated on the device",
         "config": {"workload": "fp64 leapfrog %d^3, dx=0.5, dt=0.05, lambda=1, g=0.2, %d steps, "
                                "energy every %d steps" % (N, K, args.out_every),
-                   "N": N, "out_every": args.out_every, "two_step_kernel": bool(on)},
-        "roofline": roofline(kname, N, ms64, K, args.out_every, on, 8, prof),
+                   "N": N, "out_every": args.out_every, "kernel": k64},
+        "roofline": roofline(kname, N, ms64, K, args.out_every, k64["steps_per_pass"], 8, prof),
         "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
     }
     if not args.no_extra:
@@ -484,9 +520,10 @@ def main():
             "value": sites * K1 / (ms1 * 1e-3), "unit": "site-updates/s", "steps": K1,
             "ms_per_step": ms1 / K1,
             "roofline": roofline("k_step_vec<double, CoefF64, true, 2, 4, 32, 0>", N, ms1, K1,
-                                 args.out_every, False, 8, "k_step_vec_fp64_%dcube" % N)}
+                                 args.out_every, 1, 8, "k_step_vec_fp64_%dcube" % N)}
         log("fp32 %d^3" % N)
-        ms32, e32, on32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
+        ms32, e32, k32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
+        kname32, prof32 = kernel_names("float", k32["steps_per_pass"], N)
         v32 = sites * K / (ms32 * 1e-3)
         rel = np.abs(e32 - e64) / np.abs(e64)
         over = np.nonzero(rel > 1e-6)[0]
@@ -499,16 +536,16 @@ def main():
             "steps_with_dE_over_E_le_1e-6": int(K if over.size == 0 else (over[0] - 1) * args.out_every),
             "dE_over_E_vs_fp64_by_output": [float("%.3g" % r) for r in rel[::max(1, len(rel) // 10)]],
             "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
-            "roofline": roofline("k_step2<float, CoefF32, true, W, RT, LP, FAST>", N, ms32, K, args.out_every,
-                                 on32, 4, "k_step2_fp32_%dcube" % N),
+            "kernel": k32,
+            "roofline": roofline(kname32, N, ms32, K, args.out_every, k32["steps_per_pass"], 4, prof32),
         }
         N5 = 512
         log("fp64 %d^3" % N5)
-        ms5, _, on5 = run_leg(pn, N5, pn.PN_FP64, 200, 40, 200, 5, m)
-        line["fp64_512"] = {"ms_per_step": ms5 / 200,
+        ms5, _, k5 = run_leg(pn, N5, pn.PN_FP64, 200, 80, 200, 5, m)
+        kname5, prof5 = kernel_names("double", k5["steps_per_pass"], N5)
+        line["fp64_512"] = {"ms_per_step": ms5 / 200, "kernel": k5,
                             "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
-                            "roofline": roofline("k_step2<double, CoefF64, true, 4, 6, true, 3>", N5, ms5, 200,
-                                                 200, on5, 8, "k_step2_fp64_512cube")}
+                            "roofline": roofline(kname5, N5, ms5, 200, 200, k5["steps_per_pass"], 8, prof5)}
         log("precision form %d^3" % args.precision_size)
         line["precision_%d" % args.precision_size] = precision_leg(
             pn, args.precision_size, args.precision_steps, args.precision_every)

@@ -184,12 +184,19 @@ int pn_sync(pn_ctx *c);
  * on the device (pn_ic_modes_device); 0 draws them on the host (pn_ic_modes)
  * and uploads them. */
 #define PN_OPT_IC_ON_DEVICE 2
-/* PN_OPT_STEP2_ZBLOCK: planes per tile of the two-step kernel.  Reads the z
- * block in use (the shape's default until the first pn_evolve call with at
- * least 32 steps has timed its first pairs of steps with z blocks 8 ... 64
- * and kept the fastest; all give the same bits).  Setting a value > 0 fixes
- * it; 0 returns to the default and tunes again. */
+/* Multi-step kernel choice.  The first pn_evolve call with at least 64 steps
+ * times its own first steps with the two-step kernel at z blocks 8 ... 64
+ * and, where it fits (FP64 N <= 256, FP32 N <= 512, N even), the three-step
+ * kernel (triples of steps per pass over HBM) at z blocks 16 ... 64, and
+ * keeps the fastest; every choice gives the same bits.  The options read the
+ * choice in use or fix it (setting any of them times the rest again):
+ * PN_OPT_STEP2_ZBLOCK: planes per two-step tile (0: chosen by timing).
+ * PN_OPT_THREE_STEP: 1 on, 0 off, -1 chosen by timing (the default); reads 1
+ *   when pn_evolve advances triples.
+ * PN_OPT_STEP3_ZBLOCK: planes per three-step tile (0: chosen by timing). */
 #define PN_OPT_STEP2_ZBLOCK 3
+#define PN_OPT_THREE_STEP 4
+#define PN_OPT_STEP3_ZBLOCK 5
 int pn_set_option(pn_ctx *c, int option, int value);
 int pn_get_option(pn_ctx *c, int option, int *value);
 /* pn_evolve bracketed by HIP events on the context's stream; waits for the

@@ -25,6 +25,8 @@ PN_LAYOUT_REF, PN_LAYOUT_DENSE = 0, 1
 PN_OPT_TWO_STEP = 1
 PN_OPT_IC_ON_DEVICE = 2
 PN_OPT_STEP2_ZBLOCK = 3
+PN_OPT_THREE_STEP = 4
+PN_OPT_STEP3_ZBLOCK = 5
 
 
 class PnParams(C.Structure):
@@ -349,9 +351,23 @@ class Lattice:
         """SetInitialConditions (initialize==3) with a PnIcParams."""
         self._chk(self.L.pn_initial_conditions(self.h, C.byref(ic)))
 
+    def three_step(self, on=None, zblock=None):
+        """Gets or sets PN_OPT_THREE_STEP (triples of steps per pass over HBM):
+        True, False, or "auto" (chosen by timing, the default); optionally its
+        z block (PN_OPT_STEP3_ZBLOCK, 0 = by timing).  Returns whether
+        evolve advances triples."""
+        if zblock is not None:
+            self._chk(self.L.pn_set_option(self.h, PN_OPT_STEP3_ZBLOCK, int(zblock)))
+        if on is not None:
+            v = -1 if on == "auto" else int(bool(on))
+            self._chk(self.L.pn_set_option(self.h, PN_OPT_THREE_STEP, v))
+        v = C.c_int()
+        self._chk(self.L.pn_get_option(self.h, PN_OPT_THREE_STEP, C.byref(v)))
+        return bool(v.value)
+
     def step2_zblock(self, zb=None):
-        """Gets or sets PN_OPT_STEP2_ZBLOCK (planes per two-step tile; 0 tunes
-        it again on the next evolve call of at least 32 steps)."""
+        """Gets or sets PN_OPT_STEP2_ZBLOCK (planes per two-step tile; 0: chosen
+        by timing on the next evolve call of at least 64 steps)."""
         if zb is not None:
             self._chk(self.L.pn_set_option(self.h, PN_OPT_STEP2_ZBLOCK, int(zb)))
         v = C.c_int()

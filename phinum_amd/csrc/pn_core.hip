@@ -216,6 +216,22 @@ __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const
                             dk1, dd1);
 }
 
+// Three leapfrog steps per pass (step3_tile): tiles of RT rows with RT-4
+// output rows and ZB planes; six planes of RT*N values in LDS.
+template <typename T, typename C, bool POW2, int W, int RT, int FAST = DIV_FAST>
+__global__ __launch_bounds__(1024) void k_step3(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                T *__restrict__ pio, T *__restrict__ phin, int N,
+                                                int ZB, C k, T dk0, T dd0, T dk1, T dd1, T dk2, T dd2) {
+  extern __shared__ unsigned char lds_raw[];
+  const int TY = RT - 4, YT = (N + TY - 1) / TY;
+  const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int yt = tile % YT, zt = tile / YT;
+  const int z0 = zt * ZB;
+  step3_tile<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+                                  (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0, dk1,
+                                  dd1, dk2, dd2);
+}
+
 // updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.
 template <typename T>
 __global__ void k_drift(T *__restrict__ phi, const T *__restrict__ pi, long long n, T dtd) {
@@ -350,6 +366,68 @@ static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T
   hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT, LP, FAST>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
+}
+
+// Three-step tile shape (W, RT, ZB): six LDS planes of RT*N values must fit.
+static bool step3_shape(const pn_ctx *c, Step2Shape *s) {
+  const int N = c->p.N;
+  const bool dp = c->p.precision == PN_FP64;
+  if (N < 8) return false;
+  if (dp) {
+    if (N <= 256 && N % 2 == 0) *s = {2, 8, 32, 1};
+    else return false;
+  } else {
+    if (N <= 256 && N % 4 == 0) *s = {4, 12, 32, 1};
+    else if (N <= 512 && N % 4 == 0) *s = {4, 8, 32, 1};
+    else return false;
+  }
+  if (s->ZB > N) s->ZB = N;
+  const int TY = s->RT - 4, YT = (N + TY - 1) / TY;
+  if ((YT - 1) * TY + s->RT - 3 >= 2 * N) return false;
+  const size_t lds = (size_t)6 * s->RT * N * c->esize;
+  return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
+}
+
+template <typename T, typename C, bool POW2, int W, int RT, int FAST = DIV_FAST>
+static void launch_step3_t(pn_ctx *c, const C &k, int ZB, const T *d) {
+  const int N = c->p.N;
+  const int YT = (N + RT - 5) / (RT - 4), ZT = (N + ZB - 1) / ZB;
+  const size_t lds = (size_t)6 * RT * N * sizeof(T);
+  if (lds > 65536) {
+    static int raised = 0;
+    if (!raised) {
+      (void)hipFuncSetAttribute((const void *)&k_step3<T, C, POW2, W, RT, FAST>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+      raised = 1;
+    }
+  }
+  hipLaunchKernelGGL((k_step3<T, C, POW2, W, RT, FAST>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
+                     (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
+                     (T *)c->phi[c->cur ^ 1], N, ZB, k, d[0], d[1], d[2], d[3], d[4], d[5]);
+}
+
+// steps n, n+1, n+2 in one pass; Pi moves to the second buffer
+static int launch_step3(pn_ctx *c, const Step2Shape &s, const double *dtk, const double *dtd) {
+  if (c->p.precision == PN_FP64) {
+    CoefF64 k = coef64(c);
+    const double d[6] = {dtk[0], dtd[0], dtk[1], dtd[1], dtk[2], dtd[2]};
+    if (k.rk_bmr) {
+      if (c->k.a2_pow2) launch_step3_t<double, CoefF64, true, 2, 8, DIV_FAST_2OP>(c, k, s.ZB, d);
+      else launch_step3_t<double, CoefF64, false, 2, 8, DIV_FAST_2OP>(c, k, s.ZB, d);
+    } else {
+      if (c->k.a2_pow2) launch_step3_t<double, CoefF64, true, 2, 8, DIV_FAST_KC>(c, k, s.ZB, d);
+      else launch_step3_t<double, CoefF64, false, 2, 8, DIV_FAST_KC>(c, k, s.ZB, d);
+    }
+  } else {
+    CoefF32 k = coef32(c);
+    const float d[6] = {(float)dtk[0], (float)dtd[0], (float)dtk[1], (float)dtd[1], (float)dtk[2], (float)dtd[2]};
+    if (s.RT == 12) launch_step3_t<float, CoefF32, true, 4, 12>(c, k, s.ZB, d);
+    else launch_step3_t<float, CoefF32, true, 4, 8>(c, k, s.ZB, d);
+  }
+  PN_CHECK_LAUNCH(c);
+  c->cur ^= 1;
+  std::swap(c->pi, c->pi_alt);
+  return PN_OK;
 }
 
 template <int FAST>
@@ -632,54 +710,101 @@ extern "C" int pn_drift(pn_ctx *c, double dtd) {
   return PN_OK;
 }
 
-// Online choice of the two-step z block.  The first evolve call with at least
-// kTuneSteps steps advances its first pairs of steps with each candidate z
-// block in turn (every z block gives the same bits), times them with events,
-// and keeps the fastest: the number of tiles against the device's CUs, the
-// redundant halo planes and the occupancy are traded off by measurement.  A
-// candidate must beat the shape's default by 3 %.
-static constexpr int kTuneSteps = 32;
+// Online choice of the multi-step kernel and its z block.  The first evolve
+// call with at least kTuneSteps steps advances its first steps with each
+// candidate in turn: the two-step kernel with z blocks 8 ... 64 and, where it
+// fits, the three-step kernel with z blocks 16 ... 64 (all give the same
+// bits).  Each is timed with events per step over two launches, and the
+// fastest is kept; the two-step default wins near-ties (3 %).  The number of
+// tiles against the device's CUs, the recomputed halo and the occupancy are
+// traded off by measurement.  Choices the caller fixed are kept.
+static constexpr int kTuneSteps = 64;
 
-static int tune_step2_zb(pn_ctx *c, Step2Shape *s2, const double *dtk, const double *dtd, int *n) {
-  static const int cand[] = {8, 12, 16, 24, 32, 48, 64};
+static int eff_zb2(const pn_ctx *c, const Step2Shape &s) {
+  const int z = c->step2_zb_user ? c->step2_zb_user : (c->step2_zb_auto ? c->step2_zb_auto : s.ZB);
+  return std::min(z, c->p.N);
+}
+static int eff_zb3(const pn_ctx *c, const Step2Shape &s) {
+  const int z = c->step3_zb_user ? c->step3_zb_user : (c->step3_zb_auto ? c->step3_zb_auto : s.ZB);
+  return std::min(z, c->p.N);
+}
+static bool use_step3(const pn_ctx *c) {
+  Step2Shape s3;
+  return c->step2 && c->pi_alt && (c->step3_mode == 1 || (c->step3_mode < 0 && c->step3_auto)) &&
+         step3_shape(c, &s3);
+}
+
+static int tune_multistep(pn_ctx *c, const double *dtk, const double *dtd, int *n, int nsteps) {
+  struct Cand {
+    int fam, zb;
+  };
+  static const int z2[] = {8, 12, 16, 24, 32, 48, 64};
+  static const int z3[] = {16, 24, 32, 48, 64};
+  Step2Shape s2, s3;
+  const bool have2 = step2_shape(c, &s2) && c->step3_mode != 1;
+  const bool have3 = c->step3_mode != 0 && step3_shape(c, &s3);
+  std::vector<Cand> cands;
+  if (have2) {
+    if (c->step2_zb_user) cands.push_back({2, c->step2_zb_user});
+    else
+      for (int z : z2)
+        if (z <= c->p.N) cands.push_back({2, z});
+  }
+  if (have3) {
+    if (c->step3_zb_user) cands.push_back({3, c->step3_zb_user});
+    else
+      for (int z : z3)
+        if (z <= c->p.N) cands.push_back({3, z});
+  }
+  if (cands.size() < 2) {
+    c->tuned = 1;
+    return PN_OK;
+  }
+  const Cand def = have2 ? Cand{2, eff_zb2(c, s2)} : Cand{3, eff_zb3(c, s3)};
+  int need = def.fam;
+  for (const Cand &cd : cands) need += 2 * cd.fam;
+  if (need > nsteps - *n) return PN_OK;  // too few steps here: a later call tunes
   hipEvent_t e0, e1;
   PN_HIP(c, hipEventCreate(&e0));
   PN_HIP(c, hipEventCreate(&e1));
-  auto pair = [&](int zb, float *ms) -> int {
-    Step2Shape t = *s2;
-    t.ZB = std::min(zb, c->p.N);
+  // one launch of the candidate on the next steps; *ms = its time per step
+  auto run = [&](const Cand &cd, float *ms) -> int {
+    Step2Shape t = cd.fam == 2 ? s2 : s3;
+    t.ZB = std::min(cd.zb, c->p.N);
     hipError_t e = hipEventRecord(e0, c->stream);
-    int rc = e == hipSuccess ? launch_step2(c, t, dtk + *n, dtd + *n) : PN_ERR_HIP;
-    *n += 2;
+    int rc = e != hipSuccess ? PN_ERR_HIP
+             : cd.fam == 2  ? launch_step2(c, t, dtk + *n, dtd + *n)
+                            : launch_step3(c, t, dtk + *n, dtd + *n);
+    *n += cd.fam;
     if (rc == PN_OK && e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (rc == PN_OK && e == hipSuccess) e = hipEventSynchronize(e1);
     if (rc == PN_OK && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
     if (rc == PN_OK && e != hipSuccess) rc = pn_fail(c, PN_ERR_HIP, "pn_evolve: %s", hipGetErrorString(e));
+    *ms /= (float)cd.fam;
     return rc;
   };
-  const int def = s2->ZB;
   float ms = 0.f, best_ms = 0.f;
-  int rc = pair(def, &ms);  // warms the caches
-  int best = def;
-  for (int i = 0; rc == PN_OK && i < (int)(sizeof(cand) / sizeof(cand[0])); i++) {
-    if (cand[i] > c->p.N) continue;
+  int rc = run(def, &ms);  // warms the caches
+  Cand best = def;
+  for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
     float t = 1e30f;
     for (int r = 0; rc == PN_OK && r < 2; r++) {
-      rc = pair(cand[i], &ms);
+      rc = run(cands[i], &ms);
       t = std::min(t, ms);
     }
-    if (cand[i] == def) t *= 0.97f;  // the default wins near-ties
+    if (cands[i].fam == def.fam && cands[i].zb == def.zb) t *= 0.97f;
     if (rc == PN_OK && (best_ms == 0.f || t < best_ms)) {
       best_ms = t;
-      best = cand[i];
+      best = cands[i];
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != PN_OK) return rc;
-  c->step2_zb = best;
-  c->step2_tuned = 1;
-  s2->ZB = std::min(best, c->p.N);
+  c->step3_auto = best.fam == 3;
+  if (best.fam == 3) c->step3_zb_auto = best.zb;
+  else c->step2_zb_auto = best.zb;
+  c->tuned = 1;
   return PN_OK;
 }
 
@@ -690,16 +815,25 @@ extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double 
   if (c->is_view) return pn_fail(c, PN_ERR_STATE, "pn_evolve: a batch view is stepped by pn_batch_evolve");
   PN_HIP(c, hipSetDevice(c->p.device));
   int n = 0;
-  Step2Shape s2;
-  if (c->step2 && c->pi_alt && step2_shape(c, &s2)) {
-    if (c->step2_zb > 0) s2.ZB = std::min(c->step2_zb, c->p.N);
-    else if (!c->step2_tuned && nsteps >= kTuneSteps) {
-      int rc = tune_step2_zb(c, &s2, dtk, dtd, &n);
+  if (c->step2 && c->pi_alt) {
+    if (!c->tuned && nsteps >= kTuneSteps) {
+      int rc = tune_multistep(c, dtk, dtd, &n, nsteps);
       if (rc != PN_OK) return rc;
     }
-    for (; n + 1 < nsteps; n += 2) {
-      int rc = launch_step2(c, s2, dtk + n, dtd + n);
-      if (rc != PN_OK) return rc;
+    Step2Shape s2, s3;
+    if (use_step3(c) && step3_shape(c, &s3)) {
+      s3.ZB = eff_zb3(c, s3);
+      for (; n + 3 <= nsteps; n += 3) {
+        int rc = launch_step3(c, s3, dtk + n, dtd + n);
+        if (rc != PN_OK) return rc;
+      }
+    }
+    if (step2_shape(c, &s2)) {
+      s2.ZB = eff_zb2(c, s2);
+      for (; n + 2 <= nsteps; n += 2) {
+        int rc = launch_step2(c, s2, dtk + n, dtd + n);
+        if (rc != PN_OK) return rc;
+      }
     }
   }
   for (; n < nsteps; n++) {
@@ -720,10 +854,18 @@ extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
     c->ic_device = value ? 1 : 0;
     return PN_OK;
   }
-  if (option == PN_OPT_STEP2_ZBLOCK) {
+  if (option == PN_OPT_THREE_STEP) {
+    Step2Shape s3;
+    if (value > 0 && (!c->pi_alt || !step3_shape(c, &s3)))
+      return pn_fail(c, PN_ERR_STATE, "pn_set_option: no three-step kernel for this lattice");
+    c->step3_mode = value > 0 ? 1 : (value == 0 ? 0 : -1);
+    c->tuned = 0;  // the next long evolve call times the remaining choices
+    return PN_OK;
+  }
+  if (option == PN_OPT_STEP3_ZBLOCK || option == PN_OPT_STEP2_ZBLOCK) {
     if (value < 0) return pn_fail(c, PN_ERR_ARG, "pn_set_option: z block %d", value);
-    c->step2_zb = value;        // 0: the default shape, tuned again by the next evolve call
-    c->step2_tuned = value > 0;
+    (option == PN_OPT_STEP3_ZBLOCK ? c->step3_zb_user : c->step2_zb_user) = value;
+    c->tuned = 0;
     return PN_OK;
   }
   return pn_fail(c, PN_ERR_ARG, "pn_set_option: unknown option %d", option);
@@ -739,9 +881,18 @@ extern "C" int pn_get_option(pn_ctx *c, int option, int *value) {
     *value = c->ic_device;
     return PN_OK;
   }
+  if (option == PN_OPT_THREE_STEP) {
+    *value = use_step3(c) ? 1 : 0;
+    return PN_OK;
+  }
+  if (option == PN_OPT_STEP3_ZBLOCK) {
+    Step2Shape s3;
+    *value = step3_shape(c, &s3) ? eff_zb3(c, s3) : 0;
+    return PN_OK;
+  }
   if (option == PN_OPT_STEP2_ZBLOCK) {
     Step2Shape s2;
-    *value = c->step2_zb > 0 ? c->step2_zb : (step2_shape(c, &s2) ? s2.ZB : 0);
+    *value = step2_shape(c, &s2) ? eff_zb2(c, s2) : 0;
     return PN_OK;
   }
   return pn_fail(c, PN_ERR_ARG, "pn_get_option: unknown option %d", option);

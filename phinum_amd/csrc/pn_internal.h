@@ -34,8 +34,13 @@ struct pn_ctx {
   int step2;           // pn_evolve advances pairs of steps in one pass
   int lds_max;         // LDS bytes a block may use on this device
   int ic_device = 1;   // initial-condition modes drawn on the device (else on the host)
-  int step2_zb = 0;    // two-step z block: 0 = the shape's default until tuned
-  int step2_tuned = 0; // z block chosen by timing (or fixed by the caller)
+  // multi-step kernels: the caller's choices (0 / -1: automatic) and the ones
+  // the first long evolve call made by timing (pn_core.hip, tune_multistep)
+  int step3_mode = -1;   // three-step kernel: -1 automatic, 0 off, 1 on
+  int step2_zb_user = 0, step3_zb_user = 0;  // z blocks fixed by the caller
+  int step2_zb_auto = 0, step3_zb_auto = 0;  // z blocks chosen by timing
+  int step3_auto = 0;    // timing chose the three-step kernel
+  int tuned = 0;         // the timing has run
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch

@@ -305,3 +305,166 @@ __device__ __forceinline__ void step2_tile(const T *__restrict__ phi, const T *_
     an_c = an_p;
   }
 }
+
+// Three leapfrog steps on one tile of full x rows: step2_tile with a third
+// stage.  Thread (tx, ty) owns W sites of row y0-2+ty; stage A advances step
+// n on all RT rows, stage B step n+1 on rows 1..RT-2, stage C step n+2 on the
+// output rows 2..RT-3.  The z loop runs A on plane za, B on za-1 and C on
+// za-2, plus two extra planes at each end of [z0, z1) for A and one for B.
+// LDS holds two planes each of A's and B's results and of Phi (six planes of
+// RT*N values); stage B reads A's plane written an iteration earlier and C
+// reads B's likewise, so one barrier per plane still orders every LDS write
+// before its reads and every read before the buffer is written again.  Halo
+// rows and planes are recomputed by neighbouring tiles with the same
+// arithmetic: the result is bit-identical to three single steps.
+template <typename T, typename C, bool POW2, int W, int FAST = DIV_FAST>
+__device__ __forceinline__ void step3_tile(const T *__restrict__ phi, const T *__restrict__ pi,
+                                           T *__restrict__ pio, T *__restrict__ phin, T *lds_g, int N,
+                                           int RT, int tx, int ty, int y0, int z0, int z1, const C &k,
+                                           T dk0, T dd0, T dk1, T dd1, T dk2, T dd2) {
+  typedef typename VecT<T, W>::type V;
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(3))) V LV;
+  LT *lds = (LT *)lds_g;
+  const size_t PL = (size_t)RT * N;  // one LDS plane
+  LT *bufA = lds, *bufB = lds + 2 * PL, *lphi = lds + 4 * PL;
+  const int x = W * tx;
+  int y = y0 - 2 + ty;
+  y = y < 0 ? y + N : (y >= N ? y - N : y);
+  const bool brow = ty >= 1 && ty <= RT - 2;
+  const bool crow = ty >= 2 && ty <= RT - 3 && (y0 + ty - 2) < N;
+  const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
+  auto slot = [z0](int z) { return (size_t)((z - z0 + 4) & 1); };
+  const long long NP = (long long)N * P;
+  // plane offsets of za-2 .. za+2, advanced by one plane per iteration
+  long long om2 = (long long)wz(z0 - 4) * P, om1 = (long long)wz(z0 - 3) * P,
+            o0 = (long long)wz(z0 - 2) * P, o1 = (long long)wz(z0 - 1) * P,
+            o2 = (long long)wz(z0) * P;
+  V an_m = *(const V *)&phi[om1 + row + x];
+  V an_c = *(const V *)&phi[o0 + row + x];
+  V an_n = *(const V *)&phi[o1 + row + x];
+  V p_n = *(const V *)&pi[o0 + row + x];
+  *(LV *)&lphi[slot(z0 - 2) * PL + ty * N + x] = an_c;
+  __syncthreads();
+  V b_m = an_m, b_c = an_c, pih = an_c, c_m = an_c, c_c = an_c, qh = an_c;
+  for (int za = z0 - 2; za <= z1 + 1; za++) {
+    V an_p = an_n, p = p_n, f1 = {}, f2 = {}, q = {};
+    // stage A, plane za, all rows
+    {
+      *(LV *)&lphi[slot(za + 1) * PL + ty * N + x] = an_p;
+      const LT *lp = lphi + slot(za) * PL;
+      const T vxp = lp[ty * N + xp], vxm = lp[ty * N + xm];
+      const V vyp = ty + 1 < RT ? *(const LV *)&lp[(ty + 1) * N + x] : *(const V *)&phi[o0 + rowp + x];
+      const V vym = ty > 0 ? *(const LV *)&lp[(ty - 1) * N + x] : *(const V *)&phi[o0 + rowm + x];
+      auto stage_a = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? vxp : an_c[(j + 1) % W];
+          const T left = (j == 0) ? vxm : an_c[(j + W - 1) % W];
+          T pj = p[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], an_p[j], an_m[j],
+                                                        an_c[j], pj, dk0);
+          else
+            site_update_f32(k, right, left, vyp[j], vym[j], an_p[j], an_m[j], an_c[j], pj, dk0);
+          p[j] = pj;
+          f1[j] = an_c[j] + pj * dd0;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(an_c[j]);
+        with_div_mode<FAST>(k, ok, stage_a);
+      } else {
+        stage_a(std::integral_constant<int, DIV_IEEE>());
+      }
+      *(LV *)&bufA[slot(za) * PL + ty * N + x] = f1;
+    }
+    // stage B, plane za-1, rows 1..RT-2
+    if (za >= z0 && brow) {
+      const LT *ba = bufA + slot(za - 1) * PL;
+      const T bxp = ba[ty * N + xp], bxm = ba[ty * N + xm];
+      const V byp = *(const LV *)&ba[(ty + 1) * N + x], bym = *(const LV *)&ba[(ty - 1) * N + x];
+      q = pih;
+      auto stage_b = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? bxp : b_c[(j + 1) % W];
+          const T left = (j == 0) ? bxm : b_c[(j + W - 1) % W];
+          T qj = q[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, byp[j], bym[j], f1[j], b_m[j],
+                                                        b_c[j], qj, dk1);
+          else
+            site_update_f32(k, right, left, byp[j], bym[j], f1[j], b_m[j], b_c[j], qj, dk1);
+          q[j] = qj;
+          f2[j] = b_c[j] + qj * dd1;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(b_c[j]);
+        with_div_mode<FAST>(k, ok, stage_b);
+      } else {
+        stage_b(std::integral_constant<int, DIV_IEEE>());
+      }
+      *(LV *)&bufB[slot(za - 1) * PL + ty * N + x] = f2;
+    }
+    // stage C, plane za-2, output rows
+    if (za >= z0 + 2 && crow) {
+      const LT *bb = bufB + slot(za - 2) * PL;
+      const T cxp = bb[ty * N + xp], cxm = bb[ty * N + xm];
+      const V cyp = *(const LV *)&bb[(ty + 1) * N + x], cym = *(const LV *)&bb[(ty - 1) * N + x];
+      V r = qh, f3;
+      auto stage_c = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? cxp : c_c[(j + 1) % W];
+          const T left = (j == 0) ? cxm : c_c[(j + W - 1) % W];
+          T rj = r[j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, cyp[j], cym[j], f2[j], c_m[j],
+                                                        c_c[j], rj, dk2);
+          else
+            site_update_f32(k, right, left, cyp[j], cym[j], f2[j], c_m[j], c_c[j], rj, dk2);
+          r[j] = rj;
+          f3[j] = c_c[j] + rj * dd2;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(c_c[j]);
+        with_div_mode<FAST>(k, ok, stage_c);
+      } else {
+        stage_c(std::integral_constant<int, DIV_IEEE>());
+      }
+      __builtin_nontemporal_store(r, (V *)&pio[om2 + row + x]);
+      __builtin_nontemporal_store(f3, (V *)&phin[om2 + row + x]);
+    }
+    if (za <= z1) {  // Phi of plane za+2 and Pi of plane za+1 for the next iteration
+      an_n = *(const V *)&phi[o2 + row + x];
+      p_n = *(const V *)&pi[o1 + row + x];
+    }
+    om2 = om1;
+    om1 = o0;
+    o0 = o1;
+    o1 = o2;
+    o2 = o2 + P == NP ? 0 : o2 + P;
+    __syncthreads();
+    an_m = an_c;
+    an_c = an_p;
+    b_m = b_c;
+    b_c = f1;
+    pih = p;
+    c_m = c_c;
+    c_c = f2;
+    qh = q;
+  }
+}

@@ -91,27 +91,82 @@ def test_two_step_zblocks_and_tuning_bitwise(N, prec):
     # long evolve call, which times its own pairs of steps with each z block
     m = O.model(N)
     phi, pi = _fields(N, 90 + N)
-    steps = 40
+    steps = 80  # the choice is timed on calls of at least 64 steps
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
     with _lat(m, prec) as L:
         L.two_step(False)
         L.set_fields(phi, pi)
         L.evolve(dtk, dtd)
         ref = L.get_fields()
-    runs = [8, 12, 24, N, 0]  # 0: tuned by the evolve call
+    runs = [8, 12, 24, N, 0]  # 0: two- or three-step kernel and z block chosen by timing
     for zb in runs:
         with _lat(m, prec) as L:
             assert L.two_step(True)
+            if zb:
+                L.three_step(False)
             L.step2_zblock(zb)
             L.set_fields(phi, pi)
             L.evolve(dtk, dtd)
             got = L.get_fields()
             used = L.step2_zblock()
+            used3 = L.three_step()
         assert np.array_equal(_bits(got[0]), _bits(ref[0])) and np.array_equal(_bits(got[1]), _bits(ref[1])), zb
         if zb:
-            assert used == zb
+            assert used == zb and not used3
         else:
-            assert used in (8, 12, 16, 24, 32, 48, 64)
+            assert used in (8, 12, 16, 24, 32, 48, 64, N)
+
+
+@pytest.mark.parametrize("N,steps,prec,zb", [(8, 7, pn.PN_FP64, 0), (10, 8, pn.PN_FP64, 0), (16, 11, pn.PN_FP64, 0),
+                                             (34, 6, pn.PN_FP64, 8), (130, 5, pn.PN_FP64, 0),
+                                             (256, 6, pn.PN_FP64, 0), (256, 3, pn.PN_FP64, 24),
+                                             (8, 9, pn.PN_FP32, 0), (36, 7, pn.PN_FP32, 16),
+                                             (256, 6, pn.PN_FP32, 0), (260, 3, pn.PN_FP32, 0),
+                                             (512, 3, pn.PN_FP32, 0)])
+def test_three_step_kernel_bitwise_equals_single_steps(N, steps, prec, zb):
+    # three steps per pass change only the evaluation order across sites
+    m = O.model(N)
+    phi, pi = _fields(N, 170 + N)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    out = []
+    for three in (True, False):
+        with _lat(m, prec) as L:
+            if three:
+                assert L.three_step(True, zblock=zb)
+            else:
+                L.two_step(False)
+            L.set_fields(phi, pi)
+            L.evolve(dtk, dtd)
+            out.append(L.get_fields())
+    assert np.array_equal(_bits(out[0][0]), _bits(out[1][0]))
+    assert np.array_equal(_bits(out[0][1]), _bits(out[1][1]))
+    if prec == pn.PN_FP64 and N <= 34:
+        ophi, opi = O.evolve(m, phi, pi, steps)
+        assert np.array_equal(_bits(out[0][0]), _bits(ophi)) and np.array_equal(_bits(out[0][1]), _bits(opi))
+
+
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_multistep_choice_uses_only_the_callers_steps(prec):
+    # the timed choice runs inside the caller's steps: with the three-step
+    # kernel forced, and chosen by timing, a call of 70 steps from the initial
+    # condition equals 70 single steps (the choice once ran past the arrays)
+    N = 64
+    m = O.model(N)
+    ic = pn.ic_params(1234, a=m["a"], Lambda=m["Lambda"], phi0=float(np.sqrt(m["Lambda"])))
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 70)
+    out = []
+    for mode in ("single", True, "auto"):
+        with _lat(m, prec) as L:
+            L.initial_conditions(ic)
+            if mode == "single":
+                L.two_step(False)
+            else:
+                L.three_step(mode)
+            L.evolve(dtk, dtd)
+            out.append(L.get_fields())
+    for got in out[1:]:
+        assert np.array_equal(_bits(got[0]), _bits(out[0][0])) and np.array_equal(_bits(got[1]), _bits(out[0][1]))
+        assert np.all(np.isfinite(got[0]))
 
 
 def test_two_step_unavailable_is_reported():
