@@ -63,7 +63,7 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True, 
     """Seeded initialize==3 IC on the device, then `steps` timed steps in chunks
     of `out_every` with an FP64 energy between chunks (outside the timing).
     The multi-step kernel is the library's own choice (timed during a warmup of
-    at least 64 steps) unless `kernel` = (steps per pass, z block) fixes it.
+    at least 128 steps) unless `kernel` = (steps per pass, z block) fixes it.
     Returns (total event ms over `steps`, energies at each output, the kernel
     used as {"steps_per_pass", "zblock"})."""
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)
@@ -443,7 +443,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=150)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--out-every", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=5000)
@@ -541,7 +541,7 @@ def main():
         }
         N5 = 512
         log("fp64 %d^3" % N5)
-        ms5, _, k5 = run_leg(pn, N5, pn.PN_FP64, 200, 80, 200, 5, m)
+        ms5, _, k5 = run_leg(pn, N5, pn.PN_FP64, 200, 150, 200, 5, m)
         kname5, prof5 = kernel_names("double", k5["steps_per_pass"], N5)
         line["fp64_512"] = {"ms_per_step": ms5 / 200, "kernel": k5,
                             "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),

@@ -184,7 +184,7 @@ int pn_sync(pn_ctx *c);
  * on the device (pn_ic_modes_device); 0 draws them on the host (pn_ic_modes)
  * and uploads them. */
 #define PN_OPT_IC_ON_DEVICE 2
-/* Multi-step kernel choice.  The first pn_evolve call with at least 64 steps
+/* Multi-step kernel choice.  The first pn_evolve call with at least 128 steps
  * times its own first steps with the two-step kernel at z blocks 8 ... 64
  * and, where it fits (FP64 N <= 256, FP32 N <= 512, N even), the three-step
  * kernel (triples of steps per pass over HBM) at z blocks 16 ... 64, and

@@ -367,7 +367,7 @@ class Lattice:
 
     def step2_zblock(self, zb=None):
         """Gets or sets PN_OPT_STEP2_ZBLOCK (planes per two-step tile; 0: chosen
-        by timing on the next evolve call of at least 64 steps)."""
+        by timing on the next evolve call of at least 128 steps)."""
         if zb is not None:
             self._chk(self.L.pn_set_option(self.h, PN_OPT_STEP2_ZBLOCK, int(zb)))
         v = C.c_int()

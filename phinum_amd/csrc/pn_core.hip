@@ -714,11 +714,15 @@ extern "C" int pn_drift(pn_ctx *c, double dtd) {
 // call with at least kTuneSteps steps advances its first steps with each
 // candidate in turn: the two-step kernel with z blocks 8 ... 64 and, where it
 // fits, the three-step kernel with z blocks 16 ... 64 (all give the same
-// bits).  Each is timed with events per step over two launches, and the
-// fastest is kept; the two-step default wins near-ties (3 %).  The number of
+// bits).  Two interleaved rounds run every candidate twice in a row and time
+// the second launch with events (so that the caches hold what that candidate
+// leaves, not what the previous one did); each keeps its fastest time per
+// step, and the fastest candidate is kept; the two-step default wins
+// near-ties (3 %).  The number of
 // tiles against the device's CUs, the recomputed halo and the occupancy are
 // traded off by measurement.  Choices the caller fixed are kept.
-static constexpr int kTuneSteps = 64;
+static constexpr int kTuneSteps = 128;
+static constexpr int kTuneRounds = 2;
 
 static int eff_zb2(const pn_ctx *c, const Step2Shape &s) {
   const int z = c->step2_zb_user ? c->step2_zb_user : (c->step2_zb_auto ? c->step2_zb_auto : s.ZB);
@@ -762,7 +766,7 @@ static int tune_multistep(pn_ctx *c, const double *dtk, const double *dtd, int *
   }
   const Cand def = have2 ? Cand{2, eff_zb2(c, s2)} : Cand{3, eff_zb3(c, s3)};
   int need = def.fam;
-  for (const Cand &cd : cands) need += 2 * cd.fam;
+  for (const Cand &cd : cands) need += kTuneRounds * 2 * cd.fam;
   if (need > nsteps - *n) return PN_OK;  // too few steps here: a later call tunes
   hipEvent_t e0, e1;
   PN_HIP(c, hipEventCreate(&e0));
@@ -783,18 +787,21 @@ static int tune_multistep(pn_ctx *c, const double *dtk, const double *dtd, int *
     *ms /= (float)cd.fam;
     return rc;
   };
-  float ms = 0.f, best_ms = 0.f;
+  float ms = 0.f;
   int rc = run(def, &ms);  // warms the caches
-  Cand best = def;
-  for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
-    float t = 1e30f;
-    for (int r = 0; rc == PN_OK && r < 2; r++) {
-      rc = run(cands[i], &ms);
-      t = std::min(t, ms);
+  std::vector<float> t(cands.size(), 1e30f);
+  for (int r = 0; rc == PN_OK && r < kTuneRounds; r++)
+    for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
+      rc = run(cands[i], &ms);  // primes the caches for this candidate
+      if (rc == PN_OK) rc = run(cands[i], &ms);
+      t[i] = std::min(t[i], ms);
     }
-    if (cands[i].fam == def.fam && cands[i].zb == def.zb) t *= 0.97f;
-    if (rc == PN_OK && (best_ms == 0.f || t < best_ms)) {
-      best_ms = t;
+  Cand best = def;
+  float best_ms = 0.f;
+  for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
+    const float ti = (cands[i].fam == def.fam && cands[i].zb == def.zb) ? t[i] * 0.97f : t[i];
+    if (best_ms == 0.f || ti < best_ms) {
+      best_ms = ti;
       best = cands[i];
     }
   }

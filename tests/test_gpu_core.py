@@ -91,7 +91,7 @@ def test_two_step_zblocks_and_tuning_bitwise(N, prec):
     # long evolve call, which times its own pairs of steps with each z block
     m = O.model(N)
     phi, pi = _fields(N, 90 + N)
-    steps = 80  # the choice is timed on calls of at least 64 steps
+    steps = 140  # the choice is timed on calls of at least 128 steps
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
     with _lat(m, prec) as L:
         L.two_step(False)
@@ -148,12 +148,13 @@ def test_three_step_kernel_bitwise_equals_single_steps(N, steps, prec, zb):
 @pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
 def test_multistep_choice_uses_only_the_callers_steps(prec):
     # the timed choice runs inside the caller's steps: with the three-step
-    # kernel forced, and chosen by timing, a call of 70 steps from the initial
-    # condition equals 70 single steps (the choice once ran past the arrays)
+    # kernel forced, and chosen by timing, a call of 130 steps from the
+    # initial condition equals 130 single steps (the choice once ran past the
+    # arrays)
     N = 64
     m = O.model(N)
     ic = pn.ic_params(1234, a=m["a"], Lambda=m["Lambda"], phi0=float(np.sqrt(m["Lambda"])))
-    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 70)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 130)
     out = []
     for mode in ("single", True, "auto"):
         with _lat(m, prec) as L:
