@@ -186,7 +186,7 @@ int pn_sync(pn_ctx *c);
 #define PN_OPT_IC_ON_DEVICE 2
 /* Multi-step kernel choice.  The first pn_evolve call with at least 128 steps
  * times its own first steps with the two-step kernel at z blocks 8 ... 64
- * and, where it fits (FP64 N <= 256, FP32 N <= 512, N even), the three-step
+ * and, where it fits (FP64 N <= 256 even, FP32 N <= 512 divisible by 4), the three-step
  * kernel (triples of steps per pass over HBM) at z blocks 16 ... 64, and
  * keeps the fastest; every choice gives the same bits.  The options read the
  * choice in use or fix it (setting any of them times the rest again):
