@@ -216,8 +216,18 @@ __global__ __launch_bounds__(1024) void k_step2(const T *__restrict__ phi, const
                             dk1, dd1);
 }
 
-// Three leapfrog steps per pass (step3_tile): tiles of RT rows with RT-4
-// output rows and ZB planes; six planes of RT*N values in LDS.
+// LDS planes of the three-step tile: the unrolled tile (a ring of three
+// planes per buffer) where it keeps the blocks per CU of the two-plane tile;
+// the FP32 12-row tile fits two blocks per CU only with two-plane buffers
+// (74 KB and 78 registers against 110 KB and 111), and keeps them.
+template <typename T, int RT>
+constexpr int step3_planes() {
+  return (sizeof(T) == 4 && RT == 12) ? 6 : 9;
+}
+
+// Three leapfrog steps per pass (step3_tile_r3 or step3_tile): tiles of RT
+// rows with RT-4 output rows and ZB planes; step3_planes() planes of RT*N
+// values in LDS.
 template <typename T, typename C, bool POW2, int W, int RT, int FAST = DIV_FAST>
 __global__ __launch_bounds__(1024) void k_step3(const T *__restrict__ phi, const T *__restrict__ pi,
                                                 T *__restrict__ pio, T *__restrict__ phin, int N,
@@ -227,9 +237,14 @@ __global__ __launch_bounds__(1024) void k_step3(const T *__restrict__ phi, const
   const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int yt = tile % YT, zt = tile / YT;
   const int z0 = zt * ZB;
-  step3_tile<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
-                                  (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0, dk1,
-                                  dd1, dk2, dd2);
+  if constexpr (step3_planes<T, RT>() == 9)
+    step3_tile_r3<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+                                       (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0,
+                                       dk1, dd1, dk2, dd2);
+  else
+    step3_tile<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
+                                    (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0, dk1,
+                                    dd1, dk2, dd2);
 }
 
 // updateFields alone (dynamics.cpp:182-183): Phi += Pi*dtd in place.
@@ -384,7 +399,9 @@ static bool step3_shape(const pn_ctx *c, Step2Shape *s) {
   if (s->ZB > N) s->ZB = N;
   const int TY = s->RT - 4, YT = (N + TY - 1) / TY;
   if ((YT - 1) * TY + s->RT - 3 >= 2 * N) return false;
-  const size_t lds = (size_t)6 * s->RT * N * c->esize;
+  const int planes = dp ? step3_planes<double, 8>()
+                        : (s->RT == 12 ? step3_planes<float, 12>() : step3_planes<float, 8>());
+  const size_t lds = (size_t)planes * s->RT * N * c->esize;
   return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
 }
 
@@ -392,7 +409,7 @@ template <typename T, typename C, bool POW2, int W, int RT, int FAST = DIV_FAST>
 static void launch_step3_t(pn_ctx *c, const C &k, int ZB, const T *d) {
   const int N = c->p.N;
   const int YT = (N + RT - 5) / (RT - 4), ZT = (N + ZB - 1) / ZB;
-  const size_t lds = (size_t)6 * RT * N * sizeof(T);
+  const size_t lds = (size_t)step3_planes<T, RT>() * RT * N * sizeof(T);
   if (lds > 65536) {
     static int raised = 0;
     if (!raised) {

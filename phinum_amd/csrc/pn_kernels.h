@@ -468,3 +468,168 @@ __device__ __forceinline__ void step3_tile(const T *__restrict__ phi, const T *_
     qh = q;
   }
 }
+
+// The same three-step tile with the plane loop unrolled by three.  Every value
+// that passes from plane to plane lives for three iterations: a Phi plane
+// (read at za+1, za, za-1 by stage A), the Pi of a plane between its three
+// kicks (stage A at za, B at za+1, C at za+2), stage A's and stage B's
+// results (read by the next stage over three planes).  With an LDS ring of
+// three planes per buffer, the slot and the register of plane z are both
+// (z - z0 + 3) % 3, a constant in each of the three copies of the loop body,
+// so nothing is copied between iterations and no LDS slot is computed.  The
+// arithmetic and its order are those of step3_tile.  LDS: 9 planes of RT*N.
+template <typename T, typename C, bool POW2, int W, int FAST = DIV_FAST>
+__device__ __forceinline__ void step3_tile_r3(const T *__restrict__ phi, const T *__restrict__ pi,
+                                              T *__restrict__ pio, T *__restrict__ phin, T *lds_g,
+                                              int N, int RT, int tx, int ty, int y0, int z0, int z1,
+                                              const C &k, T dk0, T dd0, T dk1, T dd1, T dk2, T dd2) {
+  typedef typename VecT<T, W>::type V;
+  typedef __attribute__((address_space(3))) T LT;
+  typedef __attribute__((address_space(3))) V LV;
+  LT *lds = (LT *)lds_g;
+  const int PL = RT * N;  // one LDS plane
+  LT *lphi = lds, *bufA = lds + 3 * PL, *bufB = lds + 6 * PL;
+  const int x = W * tx;
+  int y = y0 - 2 + ty;
+  y = y < 0 ? y + N : (y >= N ? y - N : y);
+  const bool brow = ty >= 1 && ty <= RT - 2;
+  const bool crow = ty >= 2 && ty <= RT - 3 && (y0 + ty - 2) < N;
+  const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
+  const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
+  const long long P = (long long)N * N;
+  const int row = y * N, rowp = yp * N, rowm = ym * N;
+  // this thread's element and its x/y neighbours within an LDS plane
+  const int me = ty * N + x, mxp = ty * N + xp, mxm = ty * N + xm;
+  const int myp = (ty + 1) * N + x, mym = (ty - 1) * N + x;
+  auto wz = [N](int z) { return z < 0 ? z + N : (z >= N ? z - N : z); };
+  const long long NP = (long long)N * P;
+  // plane offsets of za-2 .. za+2, advanced by one plane per iteration
+  long long om2 = (long long)wz(z0 - 4) * P, om1 = (long long)wz(z0 - 3) * P,
+            o0 = (long long)wz(z0 - 2) * P, o1 = (long long)wz(z0 - 1) * P,
+            o2 = (long long)wz(z0) * P;
+  // by plane z, index (z - z0 + 3) % 3: Phi, Pi, stage A's and stage B's results
+  V F[3], Q[3], RA[3], RB[3];
+  F[0] = *(const V *)&phi[om1 + row + x];
+  F[1] = *(const V *)&phi[o0 + row + x];
+  F[2] = *(const V *)&phi[o1 + row + x];
+  Q[1] = *(const V *)&pi[o0 + row + x];
+  Q[0] = Q[2] = RA[0] = RA[1] = RA[2] = RB[0] = RB[1] = RB[2] = F[1];
+  *(LV *)&lphi[PL + me] = F[1];
+  __syncthreads();
+  // one iteration at plane za; S = (za - z0 + 2) % 3
+  auto iter = [&](auto sc, int za) {
+    constexpr int S = decltype(sc)::value;
+    constexpr int c = (S + 1) % 3, m = S % 3, n = (S + 2) % 3;  // za; za-1, za+2; za+1, za-2
+    // stage A, plane za, all rows: Pi of za in Q[c], result in RA[c]
+    {
+      *(LV *)&lphi[n * PL + me] = F[n];
+      const LT *lp = lphi + c * PL;
+      const T vxp = lp[mxp], vxm = lp[mxm];
+      const V vyp = ty + 1 < RT ? *(const LV *)&lp[myp] : *(const V *)&phi[o0 + rowp + x];
+      const V vym = ty > 0 ? *(const LV *)&lp[mym] : *(const V *)&phi[o0 + rowm + x];
+      auto stage_a = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? vxp : F[c][(j + 1) % W];
+          const T left = (j == 0) ? vxm : F[c][(j + W - 1) % W];
+          T pj = Q[c][j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, vyp[j], vym[j], F[n][j], F[m][j],
+                                                        F[c][j], pj, dk0);
+          else
+            site_update_f32(k, right, left, vyp[j], vym[j], F[n][j], F[m][j], F[c][j], pj, dk0);
+          Q[c][j] = pj;
+          RA[c][j] = F[c][j] + pj * dd0;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(F[c][j]);
+        with_div_mode<FAST>(k, ok, stage_a);
+      } else {
+        stage_a(std::integral_constant<int, DIV_IEEE>());
+      }
+      *(LV *)&bufA[c * PL + me] = RA[c];
+    }
+    // stage B, plane za-1, rows 1..RT-2: Pi of za-1 in Q[m], result in RB[m]
+    if (za >= z0 && brow) {
+      const LT *ba = bufA + m * PL;
+      const T bxp = ba[mxp], bxm = ba[mxm];
+      const V byp = *(const LV *)&ba[myp], bym = *(const LV *)&ba[mym];
+      auto stage_b = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? bxp : RA[m][(j + 1) % W];
+          const T left = (j == 0) ? bxm : RA[m][(j + W - 1) % W];
+          T qj = Q[m][j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, byp[j], bym[j], RA[c][j], RA[n][j],
+                                                        RA[m][j], qj, dk1);
+          else
+            site_update_f32(k, right, left, byp[j], bym[j], RA[c][j], RA[n][j], RA[m][j], qj, dk1);
+          Q[m][j] = qj;
+          RB[m][j] = RA[m][j] + qj * dd1;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(RA[m][j]);
+        with_div_mode<FAST>(k, ok, stage_b);
+      } else {
+        stage_b(std::integral_constant<int, DIV_IEEE>());
+      }
+      *(LV *)&bufB[m * PL + me] = RB[m];
+    }
+    // stage C, plane za-2, output rows: Pi of za-2 in Q[n]
+    if (za >= z0 + 2 && crow) {
+      const LT *bb = bufB + n * PL;
+      const T cxp = bb[mxp], cxm = bb[mxm];
+      const V cyp = *(const LV *)&bb[myp], cym = *(const LV *)&bb[mym];
+      V f3;
+      auto stage_c = [&](auto div) {
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          const T right = (j == W - 1) ? cxp : RB[n][(j + 1) % W];
+          const T left = (j == 0) ? cxm : RB[n][(j + W - 1) % W];
+          T rj = Q[n][j];
+          if constexpr (sizeof(T) == 8)
+            site_update_f64<POW2, decltype(div)::value>(k, right, left, cyp[j], cym[j], RB[m][j], RB[c][j],
+                                                        RB[n][j], rj, dk2);
+          else
+            site_update_f32(k, right, left, cyp[j], cym[j], RB[m][j], RB[c][j], RB[n][j], rj, dk2);
+          Q[n][j] = rj;
+          f3[j] = RB[n][j] + rj * dd2;
+        }
+      };
+      if constexpr (sizeof(T) == 8) {
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < W; j++) ok = ok && fast_div_site(RB[n][j]);
+        with_div_mode<FAST>(k, ok, stage_c);
+      } else {
+        stage_c(std::integral_constant<int, DIV_IEEE>());
+      }
+      __builtin_nontemporal_store(Q[n], (V *)&pio[om2 + row + x]);
+      __builtin_nontemporal_store(f3, (V *)&phin[om2 + row + x]);
+    }
+    if (za <= z1) {  // Phi of plane za+2 (over za-1's) and Pi of plane za+1 (over za-2's)
+      F[m] = *(const V *)&phi[o2 + row + x];
+      Q[n] = *(const V *)&pi[o1 + row + x];
+    }
+    om2 = om1;
+    om1 = o0;
+    o0 = o1;
+    o1 = o2;
+    o2 = o2 + P == NP ? 0 : o2 + P;
+    __syncthreads();
+  };
+  for (int za = z0 - 2; za <= z1 + 1; za += 3) {
+    iter(std::integral_constant<int, 0>(), za);
+    if (za + 1 > z1 + 1) break;
+    iter(std::integral_constant<int, 1>(), za + 1);
+    if (za + 2 > z1 + 1) break;
+    iter(std::integral_constant<int, 2>(), za + 2);
+  }
+}
