@@ -383,7 +383,7 @@ static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
 }
 
-// Three-step tile shape (W, RT, ZB): six LDS planes of RT*N values must fit.
+// Three-step tile shape (W, RT, ZB): step3_planes() LDS planes of RT*N values must fit.
 static bool step3_shape(const pn_ctx *c, Step2Shape *s) {
   const int N = c->p.N;
   const bool dp = c->p.precision == PN_FP64;
