@@ -63,7 +63,7 @@ def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True, 
     """Seeded initialize==3 IC on the device, then `steps` timed steps in chunks
     of `out_every` with an FP64 energy between chunks (outside the timing).
     The multi-step kernel is the library's own choice (timed during a warmup of
-    at least 128 steps) unless `kernel` = (steps per pass, z block) fixes it.
+    at least 118 steps) unless `kernel` = (steps per pass, z block) fixes it.
     Returns (total event ms over `steps`, energies at each output, the kernel
     used as {"steps_per_pass", "zblock"})."""
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)

@@ -184,19 +184,23 @@ int pn_sync(pn_ctx *c);
  * on the device (pn_ic_modes_device); 0 draws them on the host (pn_ic_modes)
  * and uploads them. */
 #define PN_OPT_IC_ON_DEVICE 2
-/* Multi-step kernel choice.  The first pn_evolve call with at least 128 steps
- * times its own first steps with the two-step kernel at z blocks 8 ... 64
- * and, where it fits (FP64 N <= 256 even, FP32 N <= 512 divisible by 4), the three-step
- * kernel (triples of steps per pass over HBM) at z blocks 16 ... 64, and
- * keeps the fastest; every choice gives the same bits.  The options read the
- * choice in use or fix it (setting any of them times the rest again):
+/* Multi-step kernel choice.  pn_evolve calls of at least 12 steps time the
+ * candidates on their own steps: the two-step kernel at z blocks 8 ... 64
+ * and, where it fits (FP64 N <= 256 even, FP32 N <= 512 divisible by 4), the
+ * three-step kernel (triples of steps per pass over HBM) at z blocks
+ * 16 ... 64, as many per call as its steps allow (up to 118 steps in all at
+ * 256^3), and the fastest is kept once every candidate has been timed;
+ * every choice gives the same bits.  The options read the choice in use or
+ * fix it (setting any of them times the rest again):
  * PN_OPT_STEP2_ZBLOCK: planes per two-step tile (0: chosen by timing).
  * PN_OPT_THREE_STEP: 1 on, 0 off, -1 chosen by timing (the default); reads 1
  *   when pn_evolve advances triples.
- * PN_OPT_STEP3_ZBLOCK: planes per three-step tile (0: chosen by timing). */
+ * PN_OPT_STEP3_ZBLOCK: planes per three-step tile (0: chosen by timing).
+ * PN_OPT_KERNEL_TIMED: read only; 1 once the timed choice has been made. */
 #define PN_OPT_STEP2_ZBLOCK 3
 #define PN_OPT_THREE_STEP 4
 #define PN_OPT_STEP3_ZBLOCK 5
+#define PN_OPT_KERNEL_TIMED 6
 int pn_set_option(pn_ctx *c, int option, int value);
 int pn_get_option(pn_ctx *c, int option, int *value);
 /* pn_evolve bracketed by HIP events on the context's stream; waits for the

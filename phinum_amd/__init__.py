@@ -27,6 +27,7 @@ PN_OPT_IC_ON_DEVICE = 2
 PN_OPT_STEP2_ZBLOCK = 3
 PN_OPT_THREE_STEP = 4
 PN_OPT_STEP3_ZBLOCK = 5
+PN_OPT_KERNEL_TIMED = 6
 
 
 class PnParams(C.Structure):
@@ -365,9 +366,16 @@ class Lattice:
         self._chk(self.L.pn_get_option(self.h, PN_OPT_THREE_STEP, C.byref(v)))
         return bool(v.value)
 
+    def kernel_timed(self):
+        """True once the timed choice of the multi-step kernel has been made
+        (PN_OPT_KERNEL_TIMED)."""
+        v = C.c_int()
+        self._chk(self.L.pn_get_option(self.h, PN_OPT_KERNEL_TIMED, C.byref(v)))
+        return bool(v.value)
+
     def step2_zblock(self, zb=None):
         """Gets or sets PN_OPT_STEP2_ZBLOCK (planes per two-step tile; 0: chosen
-        by timing on the next evolve call of at least 128 steps)."""
+        by timing on the next evolve calls of at least 12 steps)."""
         if zb is not None:
             self._chk(self.L.pn_set_option(self.h, PN_OPT_STEP2_ZBLOCK, int(zb)))
         v = C.c_int()

@@ -727,18 +727,19 @@ extern "C" int pn_drift(pn_ctx *c, double dtd) {
   return PN_OK;
 }
 
-// Online choice of the multi-step kernel and its z block.  The first evolve
-// call with at least kTuneSteps steps advances its first steps with each
-// candidate in turn: the two-step kernel with z blocks 8 ... 64 and, where it
-// fits, the three-step kernel with z blocks 16 ... 64 (all give the same
-// bits).  Two interleaved rounds run every candidate twice in a row and time
-// the second launch with events (so that the caches hold what that candidate
-// leaves, not what the previous one did); each keeps its fastest time per
-// step, and the fastest candidate is kept; the two-step default wins
-// near-ties (3 %).  The number of
-// tiles against the device's CUs, the recomputed halo and the occupancy are
-// traded off by measurement.  Choices the caller fixed are kept.
-static constexpr int kTuneSteps = 128;
+// Online choice of the multi-step kernel and its z block.  The candidates are
+// the two-step kernel with z blocks 8 ... 64 and, where it fits, the
+// three-step kernel with z blocks 16 ... 64 (all give the same bits).  Each
+// is timed with events on the caller's own steps: it runs twice in a row and
+// the second launch is timed (so that the caches hold what that candidate
+// leaves, not what the previous one did), over two interleaved rounds; each
+// keeps its fastest time per step.  The timing goes on across evolve calls,
+// as many candidates per call as its steps allow (calls of at least
+// kTuneMin steps), and once every candidate has run both rounds the fastest
+// is kept; the two-step default wins near-ties (3 %).  The number of tiles
+// against the device's CUs, the recomputed halo and the occupancy are traded
+// off by measurement.  Choices the caller fixed are kept.
+static constexpr int kTuneMin = 12;
 static constexpr int kTuneRounds = 2;
 
 static int eff_zb2(const pn_ctx *c, const Step2Shape &s) {
@@ -755,79 +756,101 @@ static bool use_step3(const pn_ctx *c) {
          step3_shape(c, &s3);
 }
 
+static void tune_reset(pn_ctx *c) {
+  c->tuned = 0;
+  c->tune_pos = 0;
+  c->tune_fam.clear();
+  c->tune_zb.clear();
+  c->tune_ms.clear();
+}
+
 static int tune_multistep(pn_ctx *c, const double *dtk, const double *dtd, int *n, int nsteps) {
-  struct Cand {
-    int fam, zb;
-  };
   static const int z2[] = {8, 12, 16, 24, 32, 48, 64};
   static const int z3[] = {16, 24, 32, 48, 64};
   Step2Shape s2, s3;
   const bool have2 = step2_shape(c, &s2) && c->step3_mode != 1;
   const bool have3 = c->step3_mode != 0 && step3_shape(c, &s3);
-  std::vector<Cand> cands;
-  if (have2) {
-    if (c->step2_zb_user) cands.push_back({2, c->step2_zb_user});
-    else
-      for (int z : z2)
-        if (z <= c->p.N) cands.push_back({2, z});
+  if (c->tune_fam.empty()) {
+    auto add = [c](int fam, int zb) {
+      c->tune_fam.push_back(fam);
+      c->tune_zb.push_back(zb);
+    };
+    if (have2) {
+      if (c->step2_zb_user) add(2, c->step2_zb_user);
+      else
+        for (int z : z2)
+          if (z <= c->p.N) add(2, z);
+    }
+    if (have3) {
+      if (c->step3_zb_user) add(3, c->step3_zb_user);
+      else
+        for (int z : z3)
+          if (z <= c->p.N) add(3, z);
+    }
+    if (c->tune_fam.size() < 2) {
+      tune_reset(c);
+      c->tuned = 1;
+      return PN_OK;
+    }
+    c->tune_ms.assign(c->tune_fam.size(), 1e30f);
+    c->tune_pos = 0;
   }
-  if (have3) {
-    if (c->step3_zb_user) cands.push_back({3, c->step3_zb_user});
-    else
-      for (int z : z3)
-        if (z <= c->p.N) cands.push_back({3, z});
-  }
-  if (cands.size() < 2) {
-    c->tuned = 1;
-    return PN_OK;
-  }
-  const Cand def = have2 ? Cand{2, eff_zb2(c, s2)} : Cand{3, eff_zb3(c, s3)};
-  int need = def.fam;
-  for (const Cand &cd : cands) need += kTuneRounds * 2 * cd.fam;
-  if (need > nsteps - *n) return PN_OK;  // too few steps here: a later call tunes
+  const int nc = (int)c->tune_fam.size();
   hipEvent_t e0, e1;
   PN_HIP(c, hipEventCreate(&e0));
-  PN_HIP(c, hipEventCreate(&e1));
-  // one launch of the candidate on the next steps; *ms = its time per step
-  auto run = [&](const Cand &cd, float *ms) -> int {
-    Step2Shape t = cd.fam == 2 ? s2 : s3;
-    t.ZB = std::min(cd.zb, c->p.N);
+  hipError_t ee = hipEventCreate(&e1);
+  if (ee != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return pn_fail(c, PN_ERR_HIP, "pn_evolve: %s", hipGetErrorString(ee));
+  }
+  // one launch of candidate i on the next steps; *ms = its time per step
+  auto run = [&](int i, float *ms) -> int {
+    const int fam = c->tune_fam[i];
+    Step2Shape t = fam == 2 ? s2 : s3;
+    t.ZB = std::min(c->tune_zb[i], c->p.N);
     hipError_t e = hipEventRecord(e0, c->stream);
     int rc = e != hipSuccess ? PN_ERR_HIP
-             : cd.fam == 2  ? launch_step2(c, t, dtk + *n, dtd + *n)
-                            : launch_step3(c, t, dtk + *n, dtd + *n);
-    *n += cd.fam;
+             : fam == 2       ? launch_step2(c, t, dtk + *n, dtd + *n)
+                              : launch_step3(c, t, dtk + *n, dtd + *n);
+    *n += fam;
     if (rc == PN_OK && e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (rc == PN_OK && e == hipSuccess) e = hipEventSynchronize(e1);
     if (rc == PN_OK && e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
     if (rc == PN_OK && e != hipSuccess) rc = pn_fail(c, PN_ERR_HIP, "pn_evolve: %s", hipGetErrorString(e));
-    *ms /= (float)cd.fam;
+    *ms /= (float)fam;
     return rc;
   };
+  int rc = PN_OK;
   float ms = 0.f;
-  int rc = run(def, &ms);  // warms the caches
-  std::vector<float> t(cands.size(), 1e30f);
-  for (int r = 0; rc == PN_OK && r < kTuneRounds; r++)
-    for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
-      rc = run(cands[i], &ms);  // primes the caches for this candidate
-      if (rc == PN_OK) rc = run(cands[i], &ms);
-      t[i] = std::min(t[i], ms);
-    }
-  Cand best = def;
-  float best_ms = 0.f;
-  for (size_t i = 0; rc == PN_OK && i < cands.size(); i++) {
-    const float ti = (cands[i].fam == def.fam && cands[i].zb == def.zb) ? t[i] * 0.97f : t[i];
-    if (best_ms == 0.f || ti < best_ms) {
-      best_ms = ti;
-      best = cands[i];
+  while (rc == PN_OK && c->tune_pos < kTuneRounds * nc) {
+    const int i = c->tune_pos % nc;
+    if (2 * c->tune_fam[i] > nsteps - *n) break;  // the next call goes on
+    rc = run(i, &ms);  // primes the caches for this candidate
+    if (rc == PN_OK) rc = run(i, &ms);
+    if (rc == PN_OK) {
+      c->tune_ms[i] = std::min(c->tune_ms[i], ms);
+      c->tune_pos++;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc != PN_OK) return rc;
-  c->step3_auto = best.fam == 3;
-  if (best.fam == 3) c->step3_zb_auto = best.zb;
-  else c->step2_zb_auto = best.zb;
+  if (rc != PN_OK || c->tune_pos < kTuneRounds * nc) return rc;
+  // the default (the shape's own or the caller's z block) wins near-ties
+  const int dfam = have2 ? 2 : 3, dzb = have2 ? eff_zb2(c, s2) : eff_zb3(c, s3);
+  int best = -1;
+  float best_ms = 0.f;
+  for (int i = 0; i < nc; i++) {
+    const bool def = c->tune_fam[i] == dfam && std::min(c->tune_zb[i], c->p.N) == dzb;
+    const float ti = def ? c->tune_ms[i] * 0.97f : c->tune_ms[i];
+    if (best < 0 || ti < best_ms) {
+      best_ms = ti;
+      best = i;
+    }
+  }
+  c->step3_auto = c->tune_fam[best] == 3;
+  if (c->tune_fam[best] == 3) c->step3_zb_auto = c->tune_zb[best];
+  else c->step2_zb_auto = c->tune_zb[best];
+  tune_reset(c);
   c->tuned = 1;
   return PN_OK;
 }
@@ -840,7 +863,7 @@ extern "C" int pn_evolve(pn_ctx *c, int nsteps, const double *dtk, const double 
   PN_HIP(c, hipSetDevice(c->p.device));
   int n = 0;
   if (c->step2 && c->pi_alt) {
-    if (!c->tuned && nsteps >= kTuneSteps) {
+    if (!c->tuned && nsteps >= kTuneMin) {
       int rc = tune_multistep(c, dtk, dtd, &n, nsteps);
       if (rc != PN_OK) return rc;
     }
@@ -883,13 +906,15 @@ extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
     if (value > 0 && (!c->pi_alt || !step3_shape(c, &s3)))
       return pn_fail(c, PN_ERR_STATE, "pn_set_option: no three-step kernel for this lattice");
     c->step3_mode = value > 0 ? 1 : (value == 0 ? 0 : -1);
-    c->tuned = 0;  // the next long evolve call times the remaining choices
+    tune_reset(c);  // the next evolve calls time the remaining choices
     return PN_OK;
   }
+  if (option == PN_OPT_KERNEL_TIMED)
+    return pn_fail(c, PN_ERR_ARG, "pn_set_option: PN_OPT_KERNEL_TIMED is read only");
   if (option == PN_OPT_STEP3_ZBLOCK || option == PN_OPT_STEP2_ZBLOCK) {
     if (value < 0) return pn_fail(c, PN_ERR_ARG, "pn_set_option: z block %d", value);
     (option == PN_OPT_STEP3_ZBLOCK ? c->step3_zb_user : c->step2_zb_user) = value;
-    c->tuned = 0;
+    tune_reset(c);
     return PN_OK;
   }
   return pn_fail(c, PN_ERR_ARG, "pn_set_option: unknown option %d", option);
@@ -897,6 +922,10 @@ extern "C" int pn_set_option(pn_ctx *c, int option, int value) {
 
 extern "C" int pn_get_option(pn_ctx *c, int option, int *value) {
   if (!c || !value) return pn_fail(c, PN_ERR_ARG, "pn_get_option: NULL argument");
+  if (option == PN_OPT_KERNEL_TIMED) {
+    *value = c->tuned;
+    return PN_OK;
+  }
   if (option == PN_OPT_TWO_STEP) {
     *value = c->step2;
     return PN_OK;

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/phinum_gpu.h"
 
@@ -41,6 +42,12 @@ struct pn_ctx {
   int step2_zb_auto = 0, step3_zb_auto = 0;  // z blocks chosen by timing
   int step3_auto = 0;    // timing chose the three-step kernel
   int tuned = 0;         // the timing has run
+  // timing in progress, carried across evolve calls: candidates as
+  // (steps per pass, z block), each one's fastest time per step so far, and
+  // the next (round, candidate) to time
+  std::vector<int> tune_fam, tune_zb;
+  std::vector<float> tune_ms;
+  int tune_pos = 0;
   int cur;             // phi[cur] holds the current field
   hipStream_t stream;
   // reduction scratch

@@ -91,7 +91,7 @@ def test_two_step_zblocks_and_tuning_bitwise(N, prec):
     # long evolve call, which times its own pairs of steps with each z block
     m = O.model(N)
     phi, pi = _fields(N, 90 + N)
-    steps = 140  # the choice is timed on calls of at least 128 steps
+    steps = 140  # the timed choice completes within these steps
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
     with _lat(m, prec) as L:
         L.two_step(False)
@@ -168,6 +168,34 @@ def test_multistep_choice_uses_only_the_callers_steps(prec):
     for got in out[1:]:
         assert np.array_equal(_bits(got[0]), _bits(out[0][0])) and np.array_equal(_bits(got[1]), _bits(out[0][1]))
         assert np.all(np.isfinite(got[0]))
+
+
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_multistep_choice_across_short_calls(prec):
+    # calls of 7 to 50 steps (the drop-in flushes at every output): the timing
+    # goes on from call to call, finishes, and every call's steps are the
+    # single steps' bits; a call shorter than 12 steps does not time
+    N = 64
+    m = O.model(N)
+    ic = pn.ic_params(1234, a=m["a"], Lambda=m["Lambda"], phi0=float(np.sqrt(m["Lambda"])))
+    calls = [7, 13, 20, 30, 12, 50, 25, 40, 33, 28, 45]
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], sum(calls))
+    with _lat(m, prec) as L:
+        L.initial_conditions(ic)
+        L.two_step(False)
+        L.evolve(dtk, dtd)
+        ref = L.get_fields()
+    with _lat(m, prec) as L:
+        L.initial_conditions(ic)
+        s = 0
+        timed = []
+        for n in calls:
+            L.evolve(dtk[s:s + n], dtd[s:s + n])
+            s += n
+            timed.append(L.kernel_timed())
+        got = L.get_fields()
+    assert not timed[0] and timed[-1]
+    assert np.array_equal(_bits(got[0]), _bits(ref[0])) and np.array_equal(_bits(got[1]), _bits(ref[1]))
 
 
 def test_two_step_unavailable_is_reported():
