@@ -251,8 +251,9 @@ def precision_leg(pn, N, steps, out_every, seed=1234):
 def batch_leg(pn, steps, out_every=500):
     """64 realizations: 32 x 64^3, 24 x 128^3, 8 x 192^3, seeds 0..63, lambda = 1,
     g drawn uniformly from [0.1, 0.4] with numpy's default_rng(0); FP64, one
-    launch per pair of steps for all of them (two-step tiles), 100-step HIP
-    graphs, tiles autotuned per size class.  Every `out_every` steps (outside
+    launch per two or three steps for all of them (multi-step tiles; the
+    autotuner keeps the faster mode and the planes per tile per size class),
+    HIP graphs of 100 or 96 steps.  Every `out_every` steps (outside
     the timing) each realization gets its energy, the false-vacuum check, its
     shell spectrum and its island catalog (energy density above 4x its mean,
     islands.cpp:783-786, 6-connected).  Returns a dict for the JSON line."""
@@ -264,8 +265,8 @@ def batch_leg(pn, steps, out_every=500):
     with pn.Batch(reals) as B:
         for r in range(len(sizes)):
             B.view(r).initial_conditions(pn.ic_params(r, a=0.5, Lambda=6.0, phi0=float(np.sqrt(6.0))))
-        two = B.two_step()  # pairs of steps per launch (default where every size fits)
-        tiles = B.autotune(trial_steps=5)
+        tiles = B.autotune(trial_steps=5)  # planes per tile, and two or three steps per launch
+        per_launch = 3 if B.three_step() else (2 if B.two_step() else 1)
         e0 = np.array([B.view(r).volume_averages().energy for r in range(len(sizes))])
         decayed = np.zeros(len(sizes), dtype=np.int32)
         total = 0.0
@@ -294,8 +295,8 @@ def batch_leg(pn, steps, out_every=500):
     sites = float(sum(N ** 3 for N in sizes))
     return {"value": sites * steps / (total * 1e-3), "unit": "site-updates/s",
             "realizations": len(sizes), "sites": int(sites), "steps": steps,
-            "ms_per_step": total / steps, "two_step": two, "tiles_rows_zblock": tiles,
-            "roofline_frac": 32.0 * sites / (total / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "ms_per_step": total / steps, "steps_per_launch": per_launch, "tiles_rows_zblock": tiles,
+            "one_step_bound_frac": 32.0 * sites / (total / steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "decayed": int(decayed.sum()),
             "energy_drift_max": float(np.max(np.abs(e1 - e0) / np.abs(e0))),
             "outputs": n_out, "analysis_s_per_output": t_analysis / max(n_out, 1),

@@ -326,13 +326,18 @@ int pn_batch_set_decayed(pn_batch *b, const int *flags);
 int pn_batch_set_tiles(pn_batch *b, int N, int rows, int zblock);
 /* Times candidate tile shapes per size class on the live batch (state is
  * restored afterwards) and keeps the fastest.  out receives nclasses triples
- * (N, rows, zblock); pass NULL to skip.  In two-step mode only the planes per
- * tile are tuned and rows is the two-step tile's thread rows. */
+ * (N, rows, zblock); pass NULL to skip.  In the multi-step modes only the
+ * planes per tile are tuned and rows is the tile's thread rows; with both
+ * multi-step modes available it tunes each and keeps the faster mode (three
+ * steps per launch on ties within 2 %). */
 int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nclasses);
-/* PN_OPT_TWO_STEP for a batch (no reference counterpart): 1 (default when
- * every realization fits: N a multiple of the vector width, N/width <= 768)
- * advances pairs of steps in one launch; 0 runs single steps.  Both give
- * bit-identical fields. */
+/* Options of a batch (no reference counterpart); all modes give bit-identical
+ * fields.  PN_OPT_TWO_STEP: 1 (default when every realization fits: N a
+ * multiple of the vector width, N/width <= 768) advances pairs of steps in
+ * one launch; 0 runs single steps (and turns three-step mode off).
+ * PN_OPT_THREE_STEP: 1 (default when every realization fits a three-step
+ * tile: N >= 8, N/width <= 768 FP64 or 1024 FP32, and the device gives a
+ * block the LDS) advances three steps in one launch; needs two-step mode. */
 int pn_batch_set_option(pn_batch *b, int option, int value);
 int pn_batch_get_option(pn_batch *b, int option, int *value);
 

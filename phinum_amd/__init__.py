@@ -602,6 +602,15 @@ class Batch:
         self._chk(self.L.pn_batch_get_option(self.h, PN_OPT_TWO_STEP, C.byref(v)))
         return bool(v.value)
 
+    def three_step(self, on=None):
+        """Gets or sets PN_OPT_THREE_STEP (three steps per launch; on by default
+        when every realization fits a three-step tile) for the batch."""
+        if on is not None:
+            self._chk(self.L.pn_batch_set_option(self.h, PN_OPT_THREE_STEP, int(bool(on))))
+        v = C.c_int()
+        self._chk(self.L.pn_batch_get_option(self.h, PN_OPT_THREE_STEP, C.byref(v)))
+        return bool(v.value)
+
     def autotune(self, trial_steps=10):
         out = np.zeros(3 * 64, dtype=np.int32)
         n = C.c_int()

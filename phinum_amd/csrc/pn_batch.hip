@@ -43,6 +43,8 @@ struct BatchDesc {
   int pow2, decayed;
   int RT2, YT2, ZB2, ZT2;  // two-step tiles: RT2 thread rows (RT2-2 output rows), ZB2 planes
   int tile_start2;         // first tile in the two-step tile list
+  int RT3, YT3, ZB3, ZT3;  // three-step tiles: RT3 thread rows (RT3-4 output rows), ZB3 planes
+  int tile_start3;         // first tile in the three-step tile list
   CoefF64 k64;
   CoefF32 k32;
 };
@@ -63,6 +65,12 @@ struct pn_batch {
   std::vector<int> htile2;
   int *dtile2 = nullptr;
   int total_tiles2 = 0;
+  int step3_ok = 0;                    // every realization fits a three-step tile
+  int step3 = 0;                       // three-step mode on (needs the two-step buffers)
+  std::map<int, int> zb3;              // N -> planes per three-step tile
+  std::vector<int> htile3;
+  int *dtile3 = nullptr;
+  int total_tiles3 = 0;
   hipStream_t stream = nullptr;
   std::vector<BatchDesc> hdesc;
   BatchDesc *ddesc = nullptr;
@@ -252,6 +260,83 @@ __global__ __launch_bounds__(kB2, 6) void k_batch_step2(const T *__restrict__ ph
   }
 }
 
+// Three steps per launch for all realizations: a grid-stride walk over the
+// three-step tile list (interleaved, so that the size classes' tiles, which
+// differ in cost, spread over the XCDs), each tile step3_tile_r3 (nine LDS
+// planes of RT3*N <= kB3*W values: one block per CU).
+// Block = kB3 = batch_b3<T>() threads, thread t -> (t % (N/W), t / (N/W)) of
+// the tile's realization; threads beyond its RT3 rows only join barriers.
+// FP64 blocks are 768 threads: with 1024 the tile spills at 128 registers;
+// 768 allows 168.
+template <typename T>
+constexpr int batch_b3() {
+  return sizeof(T) == 8 ? 768 : 1024;
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(batch_b3<T>()) void k_batch_step3(const T *__restrict__ phi_all,
+                                                     const T *__restrict__ pi_all,
+                                                     T *__restrict__ pio_all,
+                                                     T *__restrict__ phin_all,
+                                                     const BatchDesc *__restrict__ desc,
+                                                     const int *__restrict__ tile_start, int R,
+                                                     int total_tiles,
+                                                     const double *__restrict__ dtk_tab,
+                                                     const double *__restrict__ dtd_tab, int cap,
+                                                     const int *__restrict__ base_ptr, int j) {
+  typedef typename VecT<T, W>::type V;
+  extern __shared__ unsigned char lds_raw[];
+  const int step = *base_ptr + j;  // this launch advances steps `step` .. `step + 2`
+  for (int b = blockIdx.x; b < total_tiles; b += gridDim.x) {
+    int lo = 0, hi = R;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tile_start[mid] <= b) lo = mid;
+      else hi = mid;
+    }
+    const BatchDesc d = desc[lo];
+    const int t = b - d.tile_start3;
+    const int yt = t % d.YT3, zt = t / d.YT3;
+    const int N = d.N, RT = d.RT3, tpr = N / W;
+    const int tx = (int)threadIdx.x % tpr, ty = (int)threadIdx.x / tpr;
+    const bool active = ty < RT;
+    const int y0 = yt * (RT - 4), z0 = zt * d.ZB3, z1 = min(z0 + d.ZB3, N);
+    const T *phi = phi_all + d.off;
+    const T *pi = pi_all + d.off;
+    T *pio = pio_all + d.off;
+    T *phin = phin_all + d.off;
+    if (d.decayed) {
+      // frozen (main.cpp:85-89 ends a decayed run): the tile's output rows
+      // keep their Phi and Pi in the buffers the next launch reads
+      const int y = y0 + ty - 2;
+      if (active && ty >= 2 && ty <= RT - 3 && y < N) {
+        const long long P = (long long)N * N;
+        for (int z = z0; z < z1; z++) {
+          const long long i = z * P + (long long)y * N + W * tx;
+          *(V *)&phin[i] = *(const V *)&phi[i];
+          *(V *)&pio[i] = *(const V *)&pi[i];
+        }
+      }
+      continue;
+    }
+    const long long tb = (long long)lo * cap + step;
+    const T dk0 = (T)dtk_tab[tb], dd0 = (T)dtd_tab[tb], dk1 = (T)dtk_tab[tb + 1],
+            dd1 = (T)dtd_tab[tb + 1], dk2 = (T)dtk_tab[tb + 2], dd2 = (T)dtd_tab[tb + 2];
+    if constexpr (sizeof(T) == 8) {
+      if (d.pow2)
+        step3_tile_r3<T, CoefF64, true, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+                                           z0, z1, d.k64, dk0, dd0, dk1, dd1, dk2, dd2);
+      else
+        step3_tile_r3<T, CoefF64, false, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active,
+                                            y0, z0, z1, d.k64, dk0, dd0, dk1, dd1, dk2, dd2);
+    } else {
+      step3_tile_r3<T, CoefF32, true, W>(phi, pi, pio, phin, (T *)lds_raw, N, RT, tx, ty, active, y0,
+                                         z0, z1, d.k32, dk0, dd0, dk1, dd1, dk2, dd2);
+    }
+    __syncthreads();  // the next tile's prologue rewrites the LDS planes
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -276,6 +361,25 @@ static int step2_rows(const pn_batch *b, int N) {
 // four planes of the tile in LDS: two of stage A results, two of Phi
 static size_t step2_lds(const pn_batch *b) { return (size_t)4 * kB2 * vec_width(b) * b->esize; }
 
+// three-step rows per tile for an N^3 realization (0 when it does not fit):
+// at least one output row, and the tile's rows y0-2 .. y0+RT-3 wrap into
+// [0, N) with one fold
+static int b3_threads(const pn_batch *b) {
+  return b->precision == PN_FP64 ? batch_b3<double>() : batch_b3<float>();
+}
+
+static int step3_rows(const pn_batch *b, int N) {
+  const int W = vec_width(b), kB3 = b3_threads(b);
+  if (N % W != 0 || N / W > kB3 || N < 8) return 0;
+  const int rt = std::min(kB3 / (N / W), N + 4);
+  if (rt < 5) return 0;
+  const int TY = rt - 4, YT = (N + TY - 1) / TY;
+  return (YT - 1) * TY + rt - 3 < 2 * N ? rt : 0;
+}
+
+// nine planes of RT3*N <= kB3*W values (step3_tile_r3)
+static size_t step3_lds(const pn_batch *b) { return (size_t)9 * b3_threads(b) * vec_width(b) * b->esize; }
+
 // rebuild descriptors and the tile list; mask (optional) selects the
 // realizations that get tiles
 static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
@@ -283,7 +387,8 @@ static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
   b->hdesc.resize(b->R);
   b->htile.assign(b->R + 1, 0);
   b->htile2.assign(b->R + 1, 0);
-  int tiles = 0, tiles2 = 0;
+  b->htile3.assign(b->R + 1, 0);
+  int tiles = 0, tiles2 = 0, tiles3 = 0;
   for (int r = 0; r < b->R; r++) {
     const pn_realization &q = b->re[r];
     BatchDesc &d = b->hdesc[r];
@@ -323,14 +428,28 @@ static int upload_desc(pn_batch *b, const std::vector<int> *mask = nullptr) {
     d.tile_start2 = tiles2;
     b->htile2[r] = tiles2;
     if (d.RT2 && (!mask || (*mask)[r])) tiles2 += d.YT2 * d.ZT2;
+    d.RT3 = step3_rows(b, q.N);
+    if (d.RT3) {
+      auto z3 = b->zb3.find(q.N);
+      d.ZB3 = std::min(z3 != b->zb3.end() ? z3->second : 32, q.N);
+      d.YT3 = (q.N + d.RT3 - 5) / (d.RT3 - 4);
+      d.ZT3 = (q.N + d.ZB3 - 1) / d.ZB3;
+    }
+    d.tile_start3 = tiles3;
+    b->htile3[r] = tiles3;
+    if (d.RT3 && (!mask || (*mask)[r])) tiles3 += d.YT3 * d.ZT3;
   }
   b->htile[b->R] = tiles;
   b->total_tiles = tiles;
   b->htile2[b->R] = tiles2;
   b->total_tiles2 = tiles2;
+  b->htile3[b->R] = tiles3;
+  b->total_tiles3 = tiles3;
   PB_HIP(b, hipMemcpyAsync(b->ddesc, b->hdesc.data(), sizeof(BatchDesc) * b->R, hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(b->dtile, b->htile.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(b->dtile2, b->htile2.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
+  if (b->dtile3)
+    PB_HIP(b, hipMemcpyAsync(b->dtile3, b->htile3.data(), sizeof(int) * (b->R + 1), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipStreamSynchronize(b->stream));
   return PN_OK;
 }
@@ -363,6 +482,7 @@ extern "C" int pn_batch_destroy(pn_batch *b) {
   (void)hipFree(b->ddesc);
   (void)hipFree(b->dtile);
   (void)hipFree(b->dtile2);
+  (void)hipFree(b->dtile3);
   (void)hipFree(b->dtk);
   (void)hipFree(b->dtd);
   (void)hipFree(b->dbase);
@@ -424,8 +544,26 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
     b->step2_ok = 0;
   }
   b->step2 = b->step2_ok;
+  // three-step mode: every realization fits a tile and the device gives a
+  // block the nine LDS planes
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    lds_max = 0;
+  b->step3_ok = b->step2_ok && step3_lds(b) <= (size_t)lds_max;
+  for (int r = 0; r < R; r++)
+    if (!step3_rows(b, b->re[r].N)) b->step3_ok = 0;
+  if (b->step3_ok) {
+    const void *kern = precision == PN_FP64 ? (const void *)&k_batch_step3<double, 2>
+                                            : (const void *)&k_batch_step3<float, 4>;
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)step3_lds(b)) != hipSuccess) {
+      (void)hipGetLastError();
+      b->step3_ok = 0;
+    }
+  }
+  b->step3 = b->step3_ok;
   if (hipMalloc(&b->ddesc, sizeof(BatchDesc) * R) != hipSuccess || hipMalloc(&b->dtile, sizeof(int) * (R + 1)) != hipSuccess ||
-      hipMalloc(&b->dtile2, sizeof(int) * (R + 1)) != hipSuccess || hipMalloc(&b->dbase, sizeof(int)) != hipSuccess)
+      hipMalloc(&b->dtile2, sizeof(int) * (R + 1)) != hipSuccess || hipMalloc(&b->dtile3, sizeof(int) * (R + 1)) != hipSuccess ||
+      hipMalloc(&b->dbase, sizeof(int)) != hipSuccess)
     return fail(PN_ERR_NOMEM, "pn_batch_create: descriptor allocation failed");
   for (int i = 0; i < 2; i++)
     if (hipMemsetAsync(b->phi[i], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
@@ -506,15 +644,41 @@ static void launch_batch_step2(pn_batch *b, int parity, int pparity, int j) {
                        b->dtile2, b->R, b->total_tiles2, b->dtk, b->dtd, b->table_cap, b->dbase, j);
 }
 
-// steps per captured graph: 100 single steps or 50 two-step launches, both
-// even, so a replay leaves the Phi parity and the Pi buffer where it found them
+// steps j .. j+2 of the tables: Phi parity -> parity^1, Pi pib[pparity] -> pib[pparity^1]
+static void launch_batch_step3(pn_batch *b, int parity, int pparity, int j) {
+  // one block per CU (nine LDS planes)
+  const int grid = std::max(1, std::min(b->total_tiles3, 256));
+  const size_t lds = step3_lds(b);
+  if (b->precision == PN_FP64)
+    hipLaunchKernelGGL((k_batch_step3<double, 2>), dim3(grid), dim3(batch_b3<double>()), lds, b->stream,
+                       (const double *)b->phi[parity], (const double *)b->pib[pparity],
+                       (double *)b->pib[pparity ^ 1], (double *)b->phi[parity ^ 1], b->ddesc,
+                       b->dtile3, b->R, b->total_tiles3, b->dtk, b->dtd, b->table_cap, b->dbase, j);
+  else
+    hipLaunchKernelGGL((k_batch_step3<float, 4>), dim3(grid), dim3(batch_b3<float>()), lds, b->stream,
+                       (const float *)b->phi[parity], (const float *)b->pib[pparity],
+                       (float *)b->pib[pparity ^ 1], (float *)b->phi[parity ^ 1], b->ddesc,
+                       b->dtile3, b->R, b->total_tiles3, b->dtk, b->dtd, b->table_cap, b->dbase, j);
+}
+
+// steps per captured graph: 100 single steps, 50 two-step launches or 32
+// three-step launches (96 steps), all even numbers of launches, so a replay
+// leaves the Phi parity and the Pi buffer where it found them
 static constexpr int kGraphLen = 100;
+static constexpr int kGraphLen3 = 96;
+static int graph_len(const pn_batch *b) { return b->step3 ? kGraphLen3 : kGraphLen; }
 
 static int ensure_graph(pn_batch *b, int parity, int pparity) {
-  if (b->gexec[parity][pparity] && b->graph_len == kGraphLen) return PN_OK;
+  const int glen = graph_len(b);
+  if (b->gexec[parity][pparity] && b->graph_len == glen) return PN_OK;
   hipGraph_t g = nullptr;
   PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  if (b->step2)
+  if (b->step3)
+    for (int j = 0; j < kGraphLen3; j += 3) {
+      const int flip = (j / 3) & 1;
+      launch_batch_step3(b, parity ^ flip, pparity ^ flip, j);
+    }
+  else if (b->step2)
     for (int j = 0; j < kGraphLen; j += 2) {
       const int flip = (j / 2) & 1;
       launch_batch_step2(b, parity ^ flip, pparity ^ flip, j);
@@ -526,7 +690,7 @@ static int ensure_graph(pn_batch *b, int parity, int pparity) {
   e = hipGraphInstantiate(&b->gexec[parity][pparity], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "pn_batch_evolve: graph instantiate: ", hipGetErrorString(e));
-  b->graph_len = kGraphLen;
+  b->graph_len = glen;
   return PN_OK;
 }
 
@@ -559,14 +723,20 @@ static int run_steps(pn_batch *b, int nsteps, int use_graph) {
   int s = 0;
   while (s < nsteps) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->dbase, s);
-    if (use_graph && nsteps - s >= kGraphLen) {
+    if (use_graph && nsteps - s >= graph_len(b)) {
       int rc = ensure_graph(b, b->cur, b->pcur);
       if (rc != PN_OK) return rc;
       PB_HIP(b, hipGraphLaunch(b->gexec[b->cur][b->pcur], b->stream));
-      s += kGraphLen;  // even length: parity and Pi buffer unchanged
+      s += graph_len(b);  // even number of launches: parity and Pi buffer unchanged
     } else {
       const int n = nsteps - s;
       int j = 0;
+      if (b->step3)
+        for (; j + 2 < n; j += 3) {
+          launch_batch_step3(b, b->cur, b->pcur, j);
+          b->cur ^= 1;
+          b->pcur ^= 1;
+        }
       if (b->step2)
         for (; j + 1 < n; j += 2) {
           launch_batch_step2(b, b->cur, b->pcur, j);
@@ -611,7 +781,7 @@ extern "C" int pn_batch_evolve_timed(pn_batch *b, int nsteps, const double *dtk,
   // tables are uploaded (and graphs captured) before the timed region
   int rc = upload_tables(b, nsteps, dtk, dtd);
   if (rc != PN_OK) return rc;
-  if (use_graph && nsteps >= kGraphLen && (rc = ensure_graph(b, b->cur, b->pcur)) != PN_OK) return rc;
+  if (use_graph && nsteps >= graph_len(b) && (rc = ensure_graph(b, b->cur, b->pcur)) != PN_OK) return rc;
   hipEvent_t e0, e1;
   PB_HIP(b, hipEventCreate(&e0));
   PB_HIP(b, hipEventCreate(&e1));
@@ -647,7 +817,7 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
   const int saved_cur = b->cur, saved_pcur = b->pcur;
   PB_HIP(b, hipMemcpyAsync(save_phi, b->phi[saved_cur], bytes, hipMemcpyDeviceToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(save_pi, b->pib[saved_pcur], bytes, hipMemcpyDeviceToDevice, b->stream));
-  if (b->step2) trial_steps += trial_steps & 1;  // whole two-step launches
+  trial_steps = (trial_steps + 5) / 6 * 6;  // whole two- and three-step launches
   std::vector<double> tk((size_t)b->R * trial_steps, 1e-3), td((size_t)b->R * trial_steps, 1e-3);
   std::vector<int> classes;
   for (auto &q : b->re)
@@ -656,36 +826,65 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
   const int rows_c[] = {2, 4, 8, 16};
   const int zb_c[] = {8, 16, 32, 64};
   int rc = PN_OK;
-  std::vector<int> results;
-  for (int N : classes) {
-    std::vector<int> mask(b->R, 0);
-    for (int r = 0; r < b->R; r++) mask[r] = b->re[r].N == N;
-    double best = 1e30;
-    std::pair<int, int> bestp = b->step2 ? std::make_pair(step2_rows(b, N), 16) : default_tiles(N);
-    // two-step mode tunes the planes per tile (its rows follow from N);
-    // single steps tune rows and planes
-    const int nrows = b->step2 ? 1 : 4;
-    for (int ri = 0; ri < nrows; ri++)
-      for (int zb : zb_c) {
-        const int ty = b->step2 ? step2_rows(b, N) : rows_c[ri];
-        if ((!b->step2 && ty > N) || zb > N) continue;
-        if (b->step2) b->zb2[N] = zb;
-        else b->tiles[N] = {ty, zb};
-        if ((rc = upload_desc(b, &mask)) != PN_OK) break;
-        double ms = 0;
-        if ((rc = pn_batch_evolve(b, trial_steps, tk.data(), td.data(), 0)) != PN_OK) break;  // warm
-        if ((rc = pn_batch_evolve_timed(b, trial_steps, tk.data(), td.data(), 0, &ms)) != PN_OK) break;
-        if (ms < best) {
-          best = ms;
-          bestp = {ty, zb};
+  // tunes every size class in the current mode; returns the sum over the
+  // classes of their fastest time (the batch's time per trial) and their choices
+  auto tune_mode = [&](std::vector<int> &results, double &total) {
+    results.clear();
+    total = 0;
+    for (int N : classes) {
+      std::vector<int> mask(b->R, 0);
+      for (int r = 0; r < b->R; r++) mask[r] = b->re[r].N == N;
+      double best = 1e30;
+      std::pair<int, int> bestp = b->step3   ? std::make_pair(step3_rows(b, N), 32)
+                                  : b->step2 ? std::make_pair(step2_rows(b, N), 16)
+                                             : default_tiles(N);
+      // multi-step modes tune the planes per tile (their rows follow from N);
+      // single steps tune rows and planes
+      const bool multi = b->step3 || b->step2;
+      const int nrows = multi ? 1 : 4;
+      for (int ri = 0; ri < nrows; ri++)
+        for (int zb : zb_c) {
+          const int ty = b->step3 ? step3_rows(b, N) : b->step2 ? step2_rows(b, N) : rows_c[ri];
+          if ((!multi && ty > N) || zb > N) continue;
+          if (b->step3) b->zb3[N] = zb;
+          else if (b->step2) b->zb2[N] = zb;
+          else b->tiles[N] = {ty, zb};
+          if ((rc = upload_desc(b, &mask)) != PN_OK) return;
+          double ms = 0;
+          if ((rc = pn_batch_evolve(b, trial_steps, tk.data(), td.data(), 0)) != PN_OK) return;  // warm
+          if ((rc = pn_batch_evolve_timed(b, trial_steps, tk.data(), td.data(), 0, &ms)) != PN_OK) return;
+          if (ms < best) {
+            best = ms;
+            bestp = {ty, zb};
+          }
         }
-      }
-    if (rc != PN_OK) break;
-    if (b->step2) b->zb2[N] = bestp.second;
-    else b->tiles[N] = bestp;
-    results.push_back(N);
-    results.push_back(bestp.first);
-    results.push_back(bestp.second);
+      if (b->step3) b->zb3[N] = bestp.second;
+      else if (b->step2) b->zb2[N] = bestp.second;
+      else b->tiles[N] = bestp;
+      total += best;
+      results.push_back(N);
+      results.push_back(bestp.first);
+      results.push_back(bestp.second);
+    }
+  };
+  std::vector<int> results;
+  double total = 0;
+  if (b->step2 && b->step3_ok) {
+    // both multi-step modes: the faster one stays on (three steps wins ties within 2 %)
+    std::vector<int> res3;
+    double t3 = 0;
+    b->step3 = 1;
+    tune_mode(res3, t3);
+    if (rc == PN_OK) {
+      b->step3 = 0;
+      tune_mode(results, total);
+    }
+    if (rc == PN_OK && t3 <= total * 1.02) {
+      b->step3 = 1;
+      results = res3;
+    }
+  } else {
+    tune_mode(results, total);
   }
   // restore the state and the full tile list
   hipError_t e = hipStreamSynchronize(b->stream);
@@ -708,6 +907,18 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
 
 extern "C" int pn_batch_set_option(pn_batch *b, int option, int value) {
   if (!b) return bfail(b, PN_ERR_ARG, "pn_batch_set_option: NULL batch");
+  if (option == PN_OPT_THREE_STEP) {
+    if (value > 0 && !(b->step3_ok && b->step2))
+      return bfail(b, PN_ERR_STATE, "pn_batch_set_option: a realization does not fit a three-step tile, "
+                                    "or two-step mode is off");
+    if ((value > 0) != (b->step3 != 0)) {
+      PB_HIP(b, hipSetDevice(b->device));
+      PB_HIP(b, hipStreamSynchronize(b->stream));
+      b->step3 = value > 0;
+      drop_graphs(b);
+    }
+    return PN_OK;
+  }
   if (option != PN_OPT_TWO_STEP) return bfail(b, PN_ERR_ARG, "pn_batch_set_option: unknown option");
   if (value && !b->step2_ok)
     return bfail(b, PN_ERR_STATE, "pn_batch_set_option: a realization does not fit a two-step tile "
@@ -716,6 +927,7 @@ extern "C" int pn_batch_set_option(pn_batch *b, int option, int value) {
     PB_HIP(b, hipSetDevice(b->device));
     PB_HIP(b, hipStreamSynchronize(b->stream));
     b->step2 = value != 0;
+    if (!b->step2) b->step3 = 0;  // three-step mode uses the two-step buffers
     drop_graphs(b);
   }
   return PN_OK;
@@ -723,6 +935,10 @@ extern "C" int pn_batch_set_option(pn_batch *b, int option, int value) {
 
 extern "C" int pn_batch_get_option(pn_batch *b, int option, int *value) {
   if (!b || !value) return bfail(b, PN_ERR_ARG, "pn_batch_get_option: NULL argument");
+  if (option == PN_OPT_THREE_STEP) {
+    *value = b->step3;
+    return PN_OK;
+  }
   if (option != PN_OPT_TWO_STEP) return bfail(b, PN_ERR_ARG, "pn_batch_get_option: unknown option");
   *value = b->step2;
   return PN_OK;

@@ -239,8 +239,8 @@ __global__ __launch_bounds__(1024) void k_step3(const T *__restrict__ phi, const
   const int z0 = zt * ZB;
   if constexpr (step3_planes<T, RT>() == 9)
     step3_tile_r3<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
-                                       (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0,
-                                       dk1, dd1, dk2, dd2);
+                                       (int)threadIdx.y, true, yt * TY, z0, min(z0 + ZB, N), k, dk0,
+                                       dd0, dk1, dd1, dk2, dd2);
   else
     step3_tile<T, C, POW2, W, FAST>(phi, pi, pio, phin, (T *)lds_raw, N, RT, (int)threadIdx.x,
                                     (int)threadIdx.y, yt * TY, z0, min(z0 + ZB, N), k, dk0, dd0, dk1,

@@ -478,11 +478,13 @@ __device__ __forceinline__ void step3_tile(const T *__restrict__ phi, const T *_
 // (z - z0 + 3) % 3, a constant in each of the three copies of the loop body,
 // so nothing is copied between iterations and no LDS slot is computed.  The
 // arithmetic and its order are those of step3_tile.  LDS: 9 planes of RT*N.
+// Threads with active == false only join the barriers (one per plane).
 template <typename T, typename C, bool POW2, int W, int FAST = DIV_FAST>
 __device__ __forceinline__ void step3_tile_r3(const T *__restrict__ phi, const T *__restrict__ pi,
                                               T *__restrict__ pio, T *__restrict__ phin, T *lds_g,
-                                              int N, int RT, int tx, int ty, int y0, int z0, int z1,
-                                              const C &k, T dk0, T dd0, T dk1, T dd1, T dk2, T dd2) {
+                                              int N, int RT, int tx, int ty, bool active, int y0, int z0,
+                                              int z1, const C &k, T dk0, T dd0, T dk1, T dd1, T dk2,
+                                              T dd2) {
   typedef typename VecT<T, W>::type V;
   typedef __attribute__((address_space(3))) T LT;
   typedef __attribute__((address_space(3))) V LV;
@@ -492,8 +494,8 @@ __device__ __forceinline__ void step3_tile_r3(const T *__restrict__ phi, const T
   const int x = W * tx;
   int y = y0 - 2 + ty;
   y = y < 0 ? y + N : (y >= N ? y - N : y);
-  const bool brow = ty >= 1 && ty <= RT - 2;
-  const bool crow = ty >= 2 && ty <= RT - 3 && (y0 + ty - 2) < N;
+  const bool brow = active && ty >= 1 && ty <= RT - 2;
+  const bool crow = active && ty >= 2 && ty <= RT - 3 && (y0 + ty - 2) < N;
   const int xp = (x + W == N) ? 0 : x + W, xm = (x == 0) ? N - 1 : x - 1;
   const int yp = (y + 1 == N) ? 0 : y + 1, ym = (y == 0) ? N - 1 : y - 1;
   const long long P = (long long)N * N;
@@ -509,19 +511,23 @@ __device__ __forceinline__ void step3_tile_r3(const T *__restrict__ phi, const T
             o2 = (long long)wz(z0) * P;
   // by plane z, index (z - z0 + 3) % 3: Phi, Pi, stage A's and stage B's results
   V F[3], Q[3], RA[3], RB[3];
-  F[0] = *(const V *)&phi[om1 + row + x];
-  F[1] = *(const V *)&phi[o0 + row + x];
-  F[2] = *(const V *)&phi[o1 + row + x];
-  Q[1] = *(const V *)&pi[o0 + row + x];
+  if (active) {
+    F[0] = *(const V *)&phi[om1 + row + x];
+    F[1] = *(const V *)&phi[o0 + row + x];
+    F[2] = *(const V *)&phi[o1 + row + x];
+    Q[1] = *(const V *)&pi[o0 + row + x];
+    *(LV *)&lphi[PL + me] = F[1];
+  } else {
+    F[0] = F[1] = F[2] = Q[1] = V{};
+  }
   Q[0] = Q[2] = RA[0] = RA[1] = RA[2] = RB[0] = RB[1] = RB[2] = F[1];
-  *(LV *)&lphi[PL + me] = F[1];
   __syncthreads();
   // one iteration at plane za; S = (za - z0 + 2) % 3
   auto iter = [&](auto sc, int za) {
     constexpr int S = decltype(sc)::value;
     constexpr int c = (S + 1) % 3, m = S % 3, n = (S + 2) % 3;  // za; za-1, za+2; za+1, za-2
     // stage A, plane za, all rows: Pi of za in Q[c], result in RA[c]
-    {
+    if (active) {
       *(LV *)&lphi[n * PL + me] = F[n];
       const LT *lp = lphi + c * PL;
       const T vxp = lp[mxp], vxm = lp[mxm];
@@ -614,7 +620,7 @@ __device__ __forceinline__ void step3_tile_r3(const T *__restrict__ phi, const T
       __builtin_nontemporal_store(Q[n], (V *)&pio[om2 + row + x]);
       __builtin_nontemporal_store(f3, (V *)&phin[om2 + row + x]);
     }
-    if (za <= z1) {  // Phi of plane za+2 (over za-1's) and Pi of plane za+1 (over za-2's)
+    if (active && za <= z1) {  // Phi of plane za+2 (over za-1's) and Pi of plane za+1 (over za-2's)
       F[m] = *(const V *)&phi[o2 + row + x];
       Q[n] = *(const V *)&pi[o1 + row + x];
     }

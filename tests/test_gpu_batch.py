@@ -81,6 +81,26 @@ def test_per_realization_time_tables():
     assert np.array_equal(a[0], oa[0]) and np.array_equal(b[0], ob[0])
 
 
+def test_batch_three_step_is_the_default_and_tunes():
+    # three steps per launch where every realization fits; the autotuner then
+    # picks planes per three-step tile and the bits stay the oracle's
+    sizes = [16, 32, 24]
+    reals = _reals(sizes, [0.1, 0.2, 0.3])
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, 200)
+    init = [_init(N, 80 + i) for i, N in enumerate(sizes)]
+    with pn.Batch(reals) as B:
+        assert B.two_step() and B.three_step()
+        for i in range(len(sizes)):
+            B.view(i).set_fields(*init[i])
+        chosen = B.autotune(trial_steps=4)
+        assert sorted(c[0] for c in chosen) == [16, 24, 32]
+        B.evolve(dtk, dtd)
+        got = [B.view(i).get_fields() for i in range(len(sizes))]
+    for i, N in enumerate(sizes):
+        ophi, opi = O.evolve(dict(reals[i]), init[i][0], init[i][1], 200)
+        assert np.array_equal(got[i][0], ophi) and np.array_equal(got[i][1], opi)
+
+
 def test_autotune_keeps_bits_and_state():
     sizes = [16, 32, 16, 24]
     reals = _reals(sizes, [0.1, 0.2, 0.3, 0.4])
@@ -159,21 +179,24 @@ def test_view_cannot_step_alone():
 @pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
 def test_batch_two_step_equals_single_steps_with_decay(prec):
     # mixed sizes, a realization frozen part-way, odd step counts, graph
-    # replays and eager launches: the two modes give the same bits
+    # replays and eager launches: three steps, two steps and single steps per
+    # launch give the same bits
     sizes = [8, 16, 12, 24, 64, 20, 36]
     reals = _reals(sizes, [0.1, 0.2, 0.3, 0.4, 0.25, 0.15, 0.35])
     init = [_init(N, 300 + i) for i, N in enumerate(sizes)]
     dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, 333)
     out = []
-    for two in (True, False):
+    for two, three in ((True, True), (True, False), (False, False)):
         with pn.Batch(reals, precision=prec) as B:
             assert B.two_step(two) == two
+            assert B.three_step(three) == three
             for i in range(len(sizes)):
                 B.view(i).set_fields(*init[i])
             B.evolve(dtk[:151], dtd[:151])
             B.set_decayed([0, 1, 0, 0, 1, 0, 0])
             B.evolve(dtk[151:], dtd[151:], use_graph=False)
             out.append([B.view(i).get_fields() for i in range(len(sizes))])
-    for i in range(len(sizes)):
-        assert np.array_equal(out[0][i][0], out[1][i][0]), i
-        assert np.array_equal(out[0][i][1], out[1][i][1]), i
+    for mode in (0, 1):
+        for i in range(len(sizes)):
+            assert np.array_equal(out[mode][i][0], out[2][i][0]), (mode, i)
+            assert np.array_equal(out[mode][i][1], out[2][i][1]), (mode, i)
