@@ -20,7 +20,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <set>
 #include <vector>
 
 #include "pn_internal.h"
@@ -364,20 +366,24 @@ static bool step2_shape(const pn_ctx *c, Step2Shape *s) {
   return (N / s->W) * s->RT <= 1024 && lds <= (size_t)c->lds_max;
 }
 
+// Raises a kernel's dynamic LDS limit to the device's maximum, once per
+// (kernel, device): the attribute is per device, and the launches of a
+// context run on its device, which is current.
+static void raise_lds_limit(const pn_ctx *c, const void *kernel) {
+  static std::mutex mu;
+  static std::set<std::pair<const void *, int>> done;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done.insert({kernel, c->p.device}).second)
+    (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
+}
+
 template <typename T, typename C, bool POW2, int W, int RT, bool LP = true, int FAST = DIV_FAST>
 static void launch_step2_t(pn_ctx *c, const C &k, int ZB, T dk0, T dd0, T dk1, T dd1) {
   const int N = c->p.N;
   const int YT = (N + RT - 3) / (RT - 2), ZT = (N + ZB - 1) / ZB;
   const size_t lds = (size_t)(LP ? 4 : 2) * RT * N * sizeof(T);
-  if (lds > 65536) {
-    // beyond 64 KB the launch needs the kernel's dynamic LDS limit raised (once)
-    static int raised = 0;
-    if (!raised) {
-      (void)hipFuncSetAttribute((const void *)&k_step2<T, C, POW2, W, RT, LP, FAST>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
-      raised = 1;
-    }
-  }
+  if (lds > 65536)  // beyond 64 KB the kernel's dynamic LDS limit is raised (once per device)
+    raise_lds_limit(c, (const void *)&k_step2<T, C, POW2, W, RT, LP, FAST>);
   hipLaunchKernelGGL((k_step2<T, C, POW2, W, RT, LP, FAST>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, dk0, dd0, dk1, dd1);
@@ -410,14 +416,7 @@ static void launch_step3_t(pn_ctx *c, const C &k, int ZB, const T *d) {
   const int N = c->p.N;
   const int YT = (N + RT - 5) / (RT - 4), ZT = (N + ZB - 1) / ZB;
   const size_t lds = (size_t)step3_planes<T, RT>() * RT * N * sizeof(T);
-  if (lds > 65536) {
-    static int raised = 0;
-    if (!raised) {
-      (void)hipFuncSetAttribute((const void *)&k_step3<T, C, POW2, W, RT, FAST>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, c->lds_max);
-      raised = 1;
-    }
-  }
+  if (lds > 65536) raise_lds_limit(c, (const void *)&k_step3<T, C, POW2, W, RT, FAST>);
   hipLaunchKernelGGL((k_step3<T, C, POW2, W, RT, FAST>), dim3(YT * ZT), dim3(N / W, RT), lds, c->stream,
                      (const T *)c->phi[c->cur], (const T *)c->pi, (T *)c->pi_alt,
                      (T *)c->phi[c->cur ^ 1], N, ZB, k, d[0], d[1], d[2], d[3], d[4], d[5]);
