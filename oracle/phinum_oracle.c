@@ -19,7 +19,9 @@
  * Parity pins (what checks each function, besides the reference build):
  *   or_drand48, or_gauss      glibc drand48 stream; polar-method identities
  *   or_update_(momenta|fields), or_evolve oracle/_ref bitwise; free-field dispersion;
- *                            homogeneous-field ODE; time reversibility
+ *                            homogeneous-field ODE (second order); time reversibility
+ *   or_evolve_f32            within binary32 rounding of or_evolve over 20 steps;
+ *                            a dropped phi^6 term or flipped phi^4 sign is not
  *   or_volume_averages       oracle/_ref bitwise; math.fsum closed forms
  *   or_extrema               oracle/_ref bitwise; numpy argmax/argmin
  *   or_bin_table             brute-force count identities (sum = N^3)
