@@ -1015,37 +1015,51 @@ __global__ __launch_bounds__(256) void k_volume(const T *__restrict__ phi, const
   Ext e;
   e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;  // fieldanalysis.cpp:324
   e.imn = e.imx = e.iab = -1;
-  const long long n3 = (long long)N * N * N;
+  // rows (y, z) walked by the grid; a block takes blockDim/N rows at a time
+  // (one when N >= blockDim), x from the thread index: no per-site division
   const long long P = (long long)N * N;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
-    const int x = (int)(i % N);
-    const int y = (int)((i / N) % N);
-    const int z = (int)(i / P);
-    const long long ixp = i - x + (x + 1 == N ? 0 : x + 1);
-    const long long iyp = i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N;
-    const long long izp = i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P;
-    const double f = (double)phi[i], p = (double)pi[i];
-    const double fx = (double)phi[ixp], fy = (double)phi[iyp], fz = (double)phi[izp];
-    double q = 0.0;
-    q += f * f;
-    two_acc(hi[0], lo[0], f);
-    two_acc(hi[1], lo[1], p);
-    two_acc(hi[2], lo[2], f * f);
-    two_acc(hi[3], lo[3], p * p);
-    two_acc(hi[4], lo[4], f * p);
-    two_acc(hi[5], lo[5], q * q);
-    two_acc(hi[6], lo[6], (q * q) * q);
-    const double dx = fx - f, dy = fy - f, dz = fz - f;
-    two_acc(hi[7], lo[7], (dx * dx / k.a2 + dy * dy / k.a2) + dz * dz / k.a2);  // :445
-    two_acc(hi[8], lo[8], k.cm * (f * f));                                        // :457
-    two_acc(hi[8], lo[8], k.lmode * ((q * q) / 24.0) + ((q * q) * q) / k.d6);     // :571
-    // extrema (fieldanalysis.cpp:331-355); AbsPhi = sqrt(0 + phi*phi)
-    const long long rp = i + P;
-    const double ab = sqrt(0.0 + f * f);
-    if (ab > e.ab) { e.ab = ab; e.iab = rp; }
-    if (f > e.mx) { e.mx = f; e.imx = rp; }
-    if (f < e.mn) { e.mn = f; e.imn = rp; }
+  const int rows = N * N;
+  const int tpr = N < (int)blockDim.x ? N : (int)blockDim.x;
+  const int rpi = (int)blockDim.x / tpr;
+  const int tr = (int)threadIdx.x / tpr, tx = (int)threadIdx.x % tpr;
+  for (int r0 = (int)blockIdx.x * rpi; r0 < rows; r0 += (int)gridDim.x * rpi) {
+    const int r = r0 + tr;
+    if (tr >= rpi || r >= rows) continue;
+    const int y = r % N, z = r / N;
+    const long long rowi = (long long)r * N;
+    const long long dyp = (long long)((y + 1 == N ? 0 : y + 1) - y) * N;
+    const long long dzp = (long long)((z + 1 == N ? 0 : z + 1) - z) * P;
+    for (int x = tx; x < N; x += tpr) {
+      const long long i = rowi + x;
+      const long long ixp = rowi + (x + 1 == N ? 0 : x + 1);
+      const double f = (double)phi[i], p = (double)pi[i];
+      const double fx = (double)phi[ixp], fy = (double)phi[i + dyp], fz = (double)phi[i + dzp];
+      double q = 0.0;
+      q += f * f;
+      two_acc(hi[0], lo[0], f);
+      two_acc(hi[1], lo[1], p);
+      two_acc(hi[2], lo[2], f * f);
+      two_acc(hi[3], lo[3], p * p);
+      two_acc(hi[4], lo[4], f * p);
+      two_acc(hi[5], lo[5], q * q);
+      two_acc(hi[6], lo[6], (q * q) * q);
+      const double dx = fx - f, dy = fy - f, dz = fz - f;
+      // :445; with a2 a power of two the division is the exact product
+      const double gx = k.pow2 ? dx * dx * k.inv_a2 : dx * dx / k.a2;
+      const double gy = k.pow2 ? dy * dy * k.inv_a2 : dy * dy / k.a2;
+      const double gz = k.pow2 ? dz * dz * k.inv_a2 : dz * dz / k.a2;
+      two_acc(hi[7], lo[7], (gx + gy) + gz);
+      two_acc(hi[8], lo[8], k.cm * (f * f));                                        // :457
+      two_acc(hi[8], lo[8], k.lmode * ((q * q) / 24.0) + ((q * q) * q) / k.d6);     // :571
+      // extrema (fieldanalysis.cpp:331-355); AbsPhi = sqrt(0 + phi*phi); a
+      // thread visits its sites in raster order, so strict comparisons keep
+      // the first index
+      const long long rp = i + P;
+      const double ab = sqrt(0.0 + f * f);
+      if (ab > e.ab) { e.ab = ab; e.iab = rp; }
+      if (f > e.mx) { e.mx = f; e.imx = rp; }
+      if (f < e.mn) { e.mn = f; e.imn = rp; }
+    }
   }
   // wavefront reduction
   for (int off = 32; off > 0; off >>= 1) {
@@ -1128,6 +1142,8 @@ static RedCoef red_coef(const pn_ctx *c) {
   k.cm = (c->p.mass_mode) * 0.5 * c->p.m2;
   k.lmode = (double)c->p.lambda_mode;
   k.d6 = 720.0 * Nc * Nc * c->p.kappa * c->p.kappa;
+  k.inv_a2 = 1.0 / k.a2;
+  k.pow2 = c->k.a2_pow2;
   return k;
 }
 

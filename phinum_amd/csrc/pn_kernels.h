@@ -135,6 +135,8 @@ struct RedCoef {
   double cm;    // (Mass_Mode*0.5)*BareM_sqr_long        (fieldanalysis.cpp:457)
   double lmode; // Lambda_Mode                           (:571)
   double d6;    // 720.0*Nc*Nc*Kappa*Kappa               (:571)
+  double inv_a2;  // 1/a2, used only when a2 is a power of two (the product is then exact)
+  int pow2;
 };
 
 
