@@ -326,12 +326,16 @@ class Lattice:
         self._chk(self.L.pn_fft_modes(self.h, int(field), _dptr(out)))
         return out.view(np.complex128).reshape(self.N, self.N, H)
 
-    def islands(self, mode, thr, capacity=65536, labels=False, grow=True, as_array=False):
+    def islands(self, mode, thr, capacity=None, labels=False, grow=True, as_array=False):
         """Island catalog (rows of PnIsland, or with as_array one numpy
         structured array over the same memory), the label array if asked for,
-        and the threshold used.  With grow, a catalog larger than `capacity`
-        is fetched again with room for all of it (the library reports the
-        count with PN_ERR_CAPACITY); without, that error is raised."""
+        and the threshold used.  capacity defaults to the last catalog's size
+        plus a quarter (at least 65536).  With grow, a catalog larger than
+        `capacity` is fetched again with room for all of it (the library
+        reports the count with PN_ERR_CAPACITY); without, that error is
+        raised."""
+        if capacity is None:
+            capacity = getattr(self, "_isl_cap", 65536)
         while True:
             arr = (PnIsland * max(capacity, 1))()
             lab = np.empty(self.N ** 3, dtype=np.int32) if labels else None
@@ -342,6 +346,7 @@ class Lattice:
                 capacity = cat.count
                 continue
             self._chk(rc)
+            self._isl_cap = max(65536, cat.count + cat.count // 4)
             if as_array:
                 rows = np.ctypeslib.as_array(arr)[:cat.count]
             else:

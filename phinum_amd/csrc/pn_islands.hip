@@ -13,12 +13,15 @@
 //   3. flatten     : label[i] = root
 //   4. numbering   : roots in increasing index order get 0,1,2,... -- exactly
 //                    the order in which the raster scan meets the islands
-//   5. statistics  : integer counts and offset sums exactly, the peak as the
-//                    largest key (phi^2 or Delta) with ties broken towards the
-//                    smallest index, and the FP64 sums (phi, potential,
-//                    kinetic and gradient energy) over each island's sites in
-//                    raster order (a stable sort groups them), compensated and
-//                    merged in a fixed order: the same bits on every call.
+//   5. statistics  : over each island's sites in raster order (a stable sort
+//                    groups them): sizes and offset sums exactly, the peak as
+//                    the largest key (phi^2 or Delta) with ties broken towards
+//                    the smallest index, and the FP64 sums (phi, potential,
+//                    kinetic and gradient energy) compensated and merged in a
+//                    fixed order: the same bits on every call.  One wave per
+//                    island up to kBig sites; larger islands in chunks of kBig
+//                    sites, one atomic per chunk for the integers and an
+//                    ordered merge of the chunks' partial sums.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -47,6 +50,11 @@ struct pn_island_state {
   int *ist = nullptr;                  // per island: first entry in the sorted sites (+ total)
   void *sort_tmp = nullptr;
   size_t sort_bytes = 0;
+  // islands above kBig sites, in chunks of kBig sorted entries
+  int *counters = nullptr;             // chunks, large islands
+  long long ccap = 0;                  // allocated chunks (large islands: half as many)
+  int *chunk_isl = nullptr, *chunk_rank = nullptr, *big_isl = nullptr, *big_c0 = nullptr;
+  double *cpart = nullptr;             // per chunk: (hi, lo) of the four sums
 };
 
 void pn_free_islands(pn_ctx *c) {
@@ -66,6 +74,12 @@ void pn_free_islands(pn_ctx *c) {
   hipFree(s->msite2);
   hipFree(s->ist);
   hipFree(s->sort_tmp);
+  hipFree(s->counters);
+  hipFree(s->chunk_isl);
+  hipFree(s->chunk_rank);
+  hipFree(s->big_isl);
+  hipFree(s->big_c0);
+  hipFree(s->cpart);
   delete s;
   c->isl = nullptr;
 }
@@ -252,33 +266,6 @@ __device__ __forceinline__ unsigned long long ord_key(double v) {
   return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
 }
 
-struct IslStatCoef {
-  double a2, cm, lmode, d6;
-};
-
-// pass A: sizes and max key (integer atomics: exact in any order)
-template <typename T>
-__global__ __launch_bounds__(kT) void k_isl_stats(const T *__restrict__ phi, const T *__restrict__ pi,
-                                                  int N, IslCoef k, const int *__restrict__ label,
-                                                  const int *__restrict__ cid,
-                                                  unsigned long long *__restrict__ ikey,
-                                                  long long *__restrict__ iint,
-                                                  double *__restrict__ idbl) {
-  const long long n3 = (long long)N * N * N, P = (long long)N * N;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
-       i += (long long)gridDim.x * kT) {
-    const int r = label[i];
-    if (r < 0) continue;
-    const int id = cid[r];
-    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    double key;
-    if (k.mode == 0) key = (double)phi[i] * (double)phi[i];
-    else key = site_delta(phi, pi, i, x, y, z, N, k);
-    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 0], 1ULL);
-    atomicMax(&ikey[id], ord_key(key));
-  }
-}
-
 // island sites per chunk, for the raster-order compaction below
 __global__ __launch_bounds__(kT) void k_isl_mcount(const int *__restrict__ label, long long n3,
                                                    int *__restrict__ mblk) {
@@ -339,101 +326,323 @@ __global__ void k_isl_starts(const int *__restrict__ key, int m, int nisl, int *
   if (p == m - 1) ist[nisl] = m;
 }
 
-// FP64 sums of an island over its sites in raster order (islands.cpp:131-142:
-// phi, potential, kinetic and gradient energy), one wave per island: lane
-// sums in (hi, lo) form merged in a fixed tree
+// x, y, z of site i (< 2^31): shifts when N is a power of two (lgN >= 0)
+__device__ __forceinline__ void site_xyz(int i, int N, int lgN, int &x, int &y, int &z) {
+  if (lgN >= 0) {
+    x = i & (N - 1);
+    y = (i >> lgN) & (N - 1);
+    z = i >> (2 * lgN);
+  } else {
+    const int r = i / N;
+    x = i - r * N;
+    y = r % N;
+    z = r / N;
+  }
+}
+
+// the peak key of a site: phi^2 (mode 0) or Delta (mode 1)
 template <typename T>
-__global__ __launch_bounds__(kT) void k_isl_sums(const T *__restrict__ phi, const T *__restrict__ pi,
-                                                 int N, IslCoef k, const int *__restrict__ ist,
-                                                 const int *__restrict__ site, int nisl,
-                                                 double *__restrict__ idbl) {
-  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
-  const long long P = (long long)N * N;
-  for (int id = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); id < nisl; id += (int)gridDim.x * wpb) {
-    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
-    const int e1 = ist[id + 1];
-    for (int e = ist[id] + lane; e < e1; e += 64) {
-      const long long i = site[e];
-      const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-      const double f = (double)phi[i], p = (double)pi[i];
-      const double fx = (double)phi[i - x + (x + 1 == N ? 0 : x + 1)];
-      const double fy = (double)phi[i + (long long)((y + 1 == N ? 0 : y + 1) - y) * N];
-      const double fz = (double)phi[i + (long long)((z + 1 == N ? 0 : z + 1) - z) * P];
-      const double grad = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / k.a2;
-      const double kin = 0.5 * (p * p);
-      const double q2 = (f * f) * (f * f);
-      const double pot = k.cm * (f * f) + k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
-      two_acc(hi[0], lo[0], f);
-      two_acc(hi[1], lo[1], pot);
-      two_acc(hi[2], lo[2], kin);
-      two_acc(hi[3], lo[3], grad);
+__device__ __forceinline__ double isl_key(const T *phi, const T *pi, int i, int N, int lgN,
+                                          const IslCoef &k) {
+  if (k.mode == 0) {
+    const double f = (double)phi[i];
+    return f * f;
+  }
+  int x, y, z;
+  site_xyz(i, N, lgN, x, y, z);
+  return site_delta(phi, pi, (long long)i, x, y, z, N, k);
+}
+
+// the summands of one site (islands.cpp:131-142): phi, potential, kinetic and
+// gradient energy density
+template <typename T>
+__device__ __forceinline__ void isl_terms(const T *phi, const T *pi, int i, int N, int lgN,
+                                          const IslCoef &k, double t[4]) {
+  int x, y, z;
+  site_xyz(i, N, lgN, x, y, z);
+  const long long P = (long long)N * N, ii = i;
+  const double f = (double)phi[ii], p = (double)pi[ii];
+  const double fx = (double)phi[ii - x + (x + 1 == N ? 0 : x + 1)];
+  const double fy = (double)phi[ii + (long long)((y + 1 == N ? 0 : y + 1) - y) * N];
+  const double fz = (double)phi[ii + (long long)((z + 1 == N ? 0 : z + 1) - z) * P];
+  t[3] = 0.5 * ((fx - f) * (fx - f) + (fy - f) * (fy - f) + (fz - f) * (fz - f)) / k.a2;
+  t[2] = 0.5 * (p * p);
+  const double q2 = (f * f) * (f * f);
+  t[1] = k.cm * (f * f) + k.lmode * (q2 / 24.0) + (q2 * (f * f) / k.d6);
+  t[0] = f;
+}
+
+// offset of site i from the peak (px, py, pz): x and y minimum image, z plain
+__device__ __forceinline__ void isl_offset(int i, int N, int lgN, int px, int py, int pz,
+                                           long long &sx, long long &sy, long long &sz) {
+  int x, y, z;
+  site_xyz(i, N, lgN, x, y, z);
+  int dx = x - px, dy = y - py;
+  if (dx >= N - N / 2) dx -= N;
+  if (dx < -(N / 2)) dx += N;
+  if (dy >= N - N / 2) dy -= N;
+  if (dy < -(N / 2)) dy += N;
+  sx += dx;
+  sy += dy;
+  sz += z - pz;
+}
+
+// a wave's largest key, its count and its first entry (entries visited in
+// increasing order per lane; kc == 0: the lane saw no entry): merged over the
+// lanes
+__device__ __forceinline__ void wave_peak(unsigned long long &km, int &kc, int &kp) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long k2 = __shfl_down(km, off, 64);
+    const int c2 = __shfl_down(kc, off, 64), p2 = __shfl_down(kp, off, 64);
+    if (c2 > 0 && (kc == 0 || k2 > km)) {
+      km = k2; kc = c2; kp = p2;
+    } else if (c2 > 0 && k2 == km) {
+      kc += c2;
+      kp = min(kp, p2);
     }
-    for (int off = 32; off > 0; off >>= 1)
+  }
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ void wave_sum4(double hi[4], double lo[4]) {
+  for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const double h2 = __shfl_down(hi[j], off, 64), l2 = __shfl_down(lo[j], off, 64);
-        two_merge(hi[j], lo[j], h2, l2);
+    for (int j = 0; j < 4; j++) {
+      const double h2 = __shfl_down(hi[j], off, 64), l2 = __shfl_down(lo[j], off, 64);
+      two_merge(hi[j], lo[j], h2, l2);
+    }
+}
+
+// Islands of up to kBig sites, one wave each, over their sites in raster
+// order (the stable sort): size, first site, peak (largest key, smallest
+// index among equal keys) and its tie count, the FP64 sums in (hi, lo) form
+// merged in a fixed tree, and the offset sums from the peak.  No atomics.
+static constexpr int kBig = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_small(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                  int N, int lgN, IslCoef k, const int *__restrict__ ist,
+                                                  const int *__restrict__ site, int nisl,
+                                                  unsigned long long *__restrict__ ikey,
+                                                  long long *__restrict__ iint,
+                                                  double *__restrict__ idbl) {
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  for (int id = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); id < nisl; id += (int)gridDim.x * wpb) {
+    const int e0 = ist[id], e1 = ist[id + 1];
+    if (e1 - e0 > kBig) continue;  // chunked: k_isl_big*
+    unsigned long long km = 0;
+    int kc = 0, kp = 0x7fffffff;
+    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const int i = site[e];
+      const unsigned long long kk = ord_key(isl_key(phi, pi, i, N, lgN, k));
+      if (kc == 0 || kk > km) {
+        km = kk; kc = 1; kp = e;
+      } else if (kk == km) {
+        kc++;
       }
-    if (lane == 0)
+      double t[4];
+      isl_terms(phi, pi, i, N, lgN, k, t);
+#pragma unroll
+      for (int j = 0; j < 4; j++) two_acc(hi[j], lo[j], t[j]);
+    }
+    wave_peak(km, kc, kp);
+    wave_sum4(hi, lo);
+    const int pk = site[__shfl(kp, 0, 64)];
+    int px, py, pz;
+    site_xyz(pk, N, lgN, px, py, pz);
+    long long sx = 0, sy = 0, sz = 0;
+    for (int e = e0 + lane; e < e1; e += 64) isl_offset(site[e], N, lgN, px, py, pz, sx, sy, sz);
+    sx = wave_sum_ll(sx);
+    sy = wave_sum_ll(sy);
+    sz = wave_sum_ll(sz);
+    if (lane == 0) {
+      long long *q = iint + (long long)id * kNI;
+      q[0] = e1 - e0;
+      q[1] = site[e0];
+      q[2] = pk;
+      q[3] = kc;
+      q[4] = sx;
+      q[5] = sy;
+      q[6] = sz;
+      ikey[id] = km;
 #pragma unroll
       for (int j = 0; j < 4; j++) idbl[(long long)id * kND + j] = hi[j] + lo[j];
+    }
   }
 }
 
-// pass B: peak = smallest index holding the max key; tie count
+// Islands above kBig sites, in chunks of kBig consecutive sorted entries
+// (counters *nchunk, *nbig; the chunks of an island are consecutive).
+// plan: one thread per island reserves its chunks
+__global__ void k_isl_bigplan(const int *__restrict__ ist, int nisl, int *__restrict__ counters,
+                              int *__restrict__ chunk_isl, int *__restrict__ chunk_rank,
+                              int *__restrict__ big_isl, int *__restrict__ big_c0) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nisl) return;
+  const int n = ist[id + 1] - ist[id];
+  if (n <= kBig) return;
+  const int nch = (n + kBig - 1) / kBig;
+  const int c0 = atomicAdd(&counters[0], nch);
+  const int b = atomicAdd(&counters[1], 1);
+  for (int c = 0; c < nch; c++) {
+    chunk_isl[c0 + c] = id;
+    chunk_rank[c0 + c] = c;
+  }
+  big_isl[b] = id;
+  big_c0[b] = c0;
+}
+
+// pass 1 per chunk: its largest key (one atomicMax per chunk) and its FP64
+// partial sums (merged in chunk order by k_isl_bigmerge)
 template <typename T>
-__global__ __launch_bounds__(kT) void k_isl_peak(const T *__restrict__ phi, const T *__restrict__ pi,
-                                                 int N, IslCoef k, const int *__restrict__ label,
-                                                 const int *__restrict__ cid,
+__global__ __launch_bounds__(kT) void k_isl_big1(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                 int N, int lgN, IslCoef k, const int *__restrict__ ist,
+                                                 const int *__restrict__ site,
+                                                 const int *__restrict__ counters,
+                                                 const int *__restrict__ chunk_isl,
+                                                 const int *__restrict__ chunk_rank,
+                                                 unsigned long long *__restrict__ ikey,
+                                                 double *__restrict__ cpart) {
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  const int nc = counters[0];
+  for (int c = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); c < nc; c += (int)gridDim.x * wpb) {
+    const int id = chunk_isl[c];
+    const int e0 = ist[id] + chunk_rank[c] * kBig, e1 = min(e0 + kBig, ist[id + 1]);
+    unsigned long long km = 0;
+    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const int i = site[e];
+      const unsigned long long kk = ord_key(isl_key(phi, pi, i, N, lgN, k));
+      km = kk > km ? kk : km;
+      double t[4];
+      isl_terms(phi, pi, i, N, lgN, k, t);
+#pragma unroll
+      for (int j = 0; j < 4; j++) two_acc(hi[j], lo[j], t[j]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long k2 = __shfl_down(km, off, 64);
+      km = k2 > km ? k2 : km;
+    }
+    wave_sum4(hi, lo);
+    if (lane == 0) {
+      atomicMax(&ikey[id], km);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        cpart[(long long)c * 8 + 2 * j] = hi[j];
+        cpart[(long long)c * 8 + 2 * j + 1] = lo[j];
+      }
+    }
+  }
+}
+
+// pass 2 per chunk: entries holding the island's largest key: their count
+// and the first of them (one atomicAdd and one atomicMin per chunk)
+template <typename T>
+__global__ __launch_bounds__(kT) void k_isl_big2(const T *__restrict__ phi, const T *__restrict__ pi,
+                                                 int N, int lgN, IslCoef k, const int *__restrict__ ist,
+                                                 const int *__restrict__ site,
+                                                 const int *__restrict__ counters,
+                                                 const int *__restrict__ chunk_isl,
+                                                 const int *__restrict__ chunk_rank,
                                                  const unsigned long long *__restrict__ ikey,
                                                  long long *__restrict__ iint) {
-  const long long n3 = (long long)N * N * N, P = (long long)N * N;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
-       i += (long long)gridDim.x * kT) {
-    const int r = label[i];
-    if (r < 0) continue;
-    const int id = cid[r];
-    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    double key;
-    if (k.mode == 0) {
-      const double f = (double)phi[i];
-      key = f * f;
-    } else {
-      key = site_delta(phi, pi, i, x, y, z, N, k);
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  const int nc = counters[0];
+  for (int c = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); c < nc; c += (int)gridDim.x * wpb) {
+    const int id = chunk_isl[c];
+    const int e0 = ist[id] + chunk_rank[c] * kBig, e1 = min(e0 + kBig, ist[id + 1]);
+    const unsigned long long km = ikey[id];
+    int kc = 0, kp = 0x7fffffff;
+    for (int e = e0 + lane; e < e1; e += 64) {
+      if (ord_key(isl_key(phi, pi, site[e], N, lgN, k)) == km) {
+        kc++;
+        kp = min(kp, e);
+      }
     }
-    if (ord_key(key) == ikey[id]) {
-      atomicMin(&iint[(long long)id * kNI + 2], (long long)i);
-      atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 3], 1ULL);
+    for (int off = 32; off > 0; off >>= 1) {
+      kc += __shfl_down(kc, off, 64);
+      kp = min(kp, __shfl_down(kp, off, 64));
+    }
+    if (lane == 0 && kc > 0) {
+      atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 3], (unsigned long long)kc);
+      atomicMin(&iint[(long long)id * kNI + 2], (long long)site[kp]);
     }
   }
 }
 
-// pass C: offsets from the peak (x, y minimum image; z plain)
-__global__ __launch_bounds__(kT) void k_isl_offsets(int N, const int *__restrict__ label,
-                                                    const int *__restrict__ cid,
-                                                    long long *__restrict__ iint,
-                                                    int *__restrict__ out_labels) {
-  const long long n3 = (long long)N * N * N, P = (long long)N * N;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
-       i += (long long)gridDim.x * kT) {
-    const int r = label[i];
-    if (out_labels) out_labels[i] = r < 0 ? -1 : cid[r];
-    if (r < 0) continue;
-    const int id = cid[r];
-    const long long pk = iint[(long long)id * kNI + 2];
-    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    const int px = (int)(pk % N), py = (int)((pk / N) % N), pz = (int)(pk / P);
-    int dx = x - px, dy = y - py;
-    if (dx >= N - N / 2) dx -= N;
-    if (dx < -(N / 2)) dx += N;
-    if (dy >= N - N / 2) dy -= N;
-    if (dy < -(N / 2)) dy += N;
-    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 4], (unsigned long long)(long long)dx);
-    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 5], (unsigned long long)(long long)dy);
-    atomicAdd((unsigned long long *)&iint[(long long)id * kNI + 6], (unsigned long long)(long long)(z - pz));
+// pass 3 per chunk: offset sums from the peak (one atomicAdd per sum and chunk)
+__global__ __launch_bounds__(kT) void k_isl_big3(int N, int lgN, const int *__restrict__ ist,
+                                                 const int *__restrict__ site,
+                                                 const int *__restrict__ counters,
+                                                 const int *__restrict__ chunk_isl,
+                                                 const int *__restrict__ chunk_rank,
+                                                 long long *__restrict__ iint) {
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  const int nc = counters[0];
+  for (int c = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); c < nc; c += (int)gridDim.x * wpb) {
+    const int id = chunk_isl[c];
+    const int e0 = ist[id] + chunk_rank[c] * kBig, e1 = min(e0 + kBig, ist[id + 1]);
+    int px, py, pz;
+    site_xyz((int)iint[(long long)id * kNI + 2], N, lgN, px, py, pz);
+    long long sx = 0, sy = 0, sz = 0;
+    for (int e = e0 + lane; e < e1; e += 64) isl_offset(site[e], N, lgN, px, py, pz, sx, sy, sz);
+    sx = wave_sum_ll(sx);
+    sy = wave_sum_ll(sy);
+    sz = wave_sum_ll(sz);
+    if (lane == 0) {
+      long long *q = iint + (long long)id * kNI;
+      atomicAdd((unsigned long long *)&q[4], (unsigned long long)sx);
+      atomicAdd((unsigned long long *)&q[5], (unsigned long long)sy);
+      atomicAdd((unsigned long long *)&q[6], (unsigned long long)sz);
+    }
   }
 }
 
+// per large island (one wave): size, first site, and the FP64 sums merged
+// from its chunks' partials in chunk order (lanes take chunks strided, then
+// a fixed tree)
+__global__ __launch_bounds__(kT) void k_isl_bigmerge(const int *__restrict__ ist,
+                                                     const int *__restrict__ site,
+                                                     const int *__restrict__ counters,
+                                                     const int *__restrict__ big_isl,
+                                                     const int *__restrict__ big_c0,
+                                                     const double *__restrict__ cpart,
+                                                     long long *__restrict__ iint,
+                                                     double *__restrict__ idbl) {
+  const int lane = (int)(threadIdx.x & 63), wpb = (int)(blockDim.x >> 6);
+  const int nb = counters[1];
+  for (int b = (int)blockIdx.x * wpb + (int)(threadIdx.x >> 6); b < nb; b += (int)gridDim.x * wpb) {
+    const int id = big_isl[b], c0 = big_c0[b];
+    const int n = ist[id + 1] - ist[id], nch = (n + kBig - 1) / kBig;
+    double hi[4] = {0.0, 0.0, 0.0, 0.0}, lo[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = lane; c < nch; c += 64)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        two_merge(hi[j], lo[j], cpart[(long long)(c0 + c) * 8 + 2 * j], cpart[(long long)(c0 + c) * 8 + 2 * j + 1]);
+    wave_sum4(hi, lo);
+    if (lane == 0) {
+      iint[(long long)id * kNI + 0] = n;
+      iint[(long long)id * kNI + 1] = site[ist[id]];
+#pragma unroll
+      for (int j = 0; j < 4; j++) idbl[(long long)id * kND + j] = hi[j] + lo[j];
+    }
+  }
+}
+
+// the label array the caller asked for: island number per site, -1 elsewhere
+__global__ __launch_bounds__(kT) void k_isl_labels(const int *__restrict__ label, long long n3,
+                                                   const int *__restrict__ cid,
+                                                   int *__restrict__ out) {
+  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3; i += (long long)gridDim.x * kT) {
+    const int r = label[i];
+    out[i] = r < 0 ? -1 : cid[r];
+  }
+}
 __global__ void k_isl_prep(int n, unsigned long long *__restrict__ ikey, long long *__restrict__ iint,
                            double *__restrict__ idbl) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -446,27 +655,18 @@ __global__ void k_isl_prep(int n, unsigned long long *__restrict__ ikey, long lo
   for (int j = 0; j < kND; j++) idbl[(long long)id * kND + j] = 0.0;
 }
 
-// first site of each island = its root
-__global__ __launch_bounds__(kT) void k_isl_roots(const int *__restrict__ label, long long n3,
-                                                  const int *__restrict__ cid,
-                                                  long long *__restrict__ iint) {
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
-       i += (long long)gridDim.x * kT)
-    if (label[i] == (int)i) iint[(long long)cid[i] * kNI + 1] = i;
-}
-
 // peak value: Phi (mode 0, islands.cpp:257) or Delta (mode 1, :260) at the peak
 template <typename T>
-__global__ void k_isl_peakval(const T *__restrict__ phi, const T *__restrict__ pi, int N,
+__global__ void k_isl_peakval(const T *__restrict__ phi, const T *__restrict__ pi, int N, int lgN,
                               IslCoef k, int n, const long long *__restrict__ iint,
                               double *__restrict__ idbl) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n) return;
-  const long long P = (long long)N * N;
-  const long long pk = iint[(long long)id * kNI + 2];
-  const int x = (int)(pk % N), y = (int)((pk / N) % N), z = (int)(pk / P);
+  const int pk = (int)iint[(long long)id * kNI + 2];
+  int x, y, z;
+  site_xyz(pk, N, lgN, x, y, z);
   idbl[(long long)id * kND + 4] =
-      k.mode == 0 ? (double)phi[pk] : site_delta(phi, pi, pk, x, y, z, N, k);
+      k.mode == 0 ? (double)phi[pk] : site_delta(phi, pi, (long long)pk, x, y, z, N, k);
 }
 
 template <typename T>
@@ -535,11 +735,29 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
     }
   }
   if (nisl > 0) {
+    // chunk tables for islands above kBig sites: fewer than 2 nsite / kBig chunks
+    const long long ccap = 2LL * nsite / kBig + 2;
+    if (ccap > s->ccap || !s->counters) {
+      hipFree(s->chunk_isl); hipFree(s->chunk_rank); hipFree(s->big_isl); hipFree(s->big_c0);
+      hipFree(s->cpart);
+      s->chunk_isl = s->chunk_rank = s->big_isl = s->big_c0 = nullptr;
+      s->cpart = nullptr;
+      s->ccap = 0;
+      if (!s->counters) PN_HIP(c, hipMalloc(&s->counters, 2 * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->chunk_isl, (size_t)ccap * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->chunk_rank, (size_t)ccap * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->big_isl, (size_t)ccap * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->big_c0, (size_t)ccap * sizeof(int)));
+      PN_HIP(c, hipMalloc(&s->cpart, (size_t)ccap * 8 * sizeof(double)));
+      s->ccap = ccap;
+    }
+    int lgN = -1;
+    if ((N & (N - 1)) == 0) {
+      lgN = 0;
+      while ((1 << lgN) < N) lgN++;
+    }
     hipLaunchKernelGGL(k_isl_prep, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, nisl,
                        s->ikey, s->iint, s->idbl);
-    hipLaunchKernelGGL(k_isl_roots, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, s->iint);
-    hipLaunchKernelGGL(k_isl_stats<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
-                       s->cid, s->ikey, s->iint, s->idbl);
     hipLaunchKernelGGL(k_isl_mscatter, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3,
                        s->mblk, s->cid, s->mkey, s->msite);
     size_t bytes = s->sort_bytes;
@@ -547,18 +765,27 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
                                                  s->msite2, nsite, 0, end_bit, c->stream));
     hipLaunchKernelGGL(k_isl_starts, dim3((nsite + 255) / 256), dim3(256), 0, c->stream, s->mkey2, nsite,
                        nisl, s->ist);
-    hipLaunchKernelGGL(k_isl_sums<T>, dim3((unsigned)std::min(4096, (nisl + 3) / 4)), dim3(kT), 0,
-                       c->stream, phi, pi, N, k, s->ist, s->msite2, nisl, s->idbl);
-    hipLaunchKernelGGL(k_isl_peak<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label,
-                       s->cid, s->ikey, s->iint);
+    PN_HIP(c, hipMemsetAsync(s->counters, 0, 2 * sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_isl_small<T>, dim3((unsigned)std::min(8192, (nisl + 3) / 4)), dim3(kT), 0,
+                       c->stream, phi, pi, N, lgN, k, s->ist, s->msite2, nisl, s->ikey, s->iint, s->idbl);
+    hipLaunchKernelGGL(k_isl_bigplan, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, s->ist, nisl,
+                       s->counters, s->chunk_isl, s->chunk_rank, s->big_isl, s->big_c0);
+    const int gbig = (int)std::max<long long>(1, std::min<long long>(2048, (ccap + 3) / 4));
+    hipLaunchKernelGGL(k_isl_big1<T>, dim3(gbig), dim3(kT), 0, c->stream, phi, pi, N, lgN, k, s->ist,
+                       s->msite2, s->counters, s->chunk_isl, s->chunk_rank, s->ikey, s->cpart);
+    hipLaunchKernelGGL(k_isl_big2<T>, dim3(gbig), dim3(kT), 0, c->stream, phi, pi, N, lgN, k, s->ist,
+                       s->msite2, s->counters, s->chunk_isl, s->chunk_rank, s->ikey, s->iint);
+    hipLaunchKernelGGL(k_isl_big3, dim3(gbig), dim3(kT), 0, c->stream, N, lgN, s->ist, s->msite2,
+                       s->counters, s->chunk_isl, s->chunk_rank, s->iint);
+    hipLaunchKernelGGL(k_isl_bigmerge, dim3(gbig), dim3(kT), 0, c->stream, s->ist, s->msite2, s->counters,
+                       s->big_isl, s->big_c0, s->cpart, s->iint, s->idbl);
     hipLaunchKernelGGL(k_isl_peakval<T>, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, phi, pi,
-                       N, k, nisl, s->iint, s->idbl);
+                       N, lgN, k, nisl, s->iint, s->idbl);
   }
   int *dlab = nullptr;
   if (out->labels) PN_HIP(c, hipMalloc(&dlab, (size_t)n3 * sizeof(int)));
-  if (nisl > 0 || dlab)
-    hipLaunchKernelGGL(k_isl_offsets, dim3(grid), dim3(kT), 0, c->stream, N, s->label, s->cid,
-                       s->iint, dlab);
+  if (dlab)
+    hipLaunchKernelGGL(k_isl_labels, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, dlab);
   hipError_t e = hipGetLastError();
   std::vector<long long> ii((size_t)nisl * kNI);
   std::vector<double> dd((size_t)nisl * kND);

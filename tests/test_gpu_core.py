@@ -497,6 +497,55 @@ def test_island_centroids_brute_force():
     assert (rows[0].cx, rows[0].cy, rows[0].cz) == (15.5, 15.5, 9.0)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_large_islands_match_bfs(mode):
+    # a threshold low enough that one island spans most of a 40^3 lattice: its
+    # statistics come from chunks of 4096 sites (one atomic per chunk for the
+    # integers, an ordered merge of the chunks' sums); the small islands from
+    # one wave each.  Catalog and labels against the oracle's BFS, centroids by
+    # brute force, and two planted equal maxima in the large island: one tie,
+    # the peak at the smaller index
+    N = 40
+    m = O.model(N)
+    P = N * N
+    phi, pi = _fields(N, 61 + mode, phi0=0.0, amp=1.0)
+    if mode == 0:
+        thr = 0.6 * np.sqrt(O.volume_averages(m, phi, pi)[2])
+        # a site of the largest island and three x neighbours after it: two
+        # equal maxima (9.0) joined through two sites of 8.0
+        cat0, lab0 = O.islands(m, 0, thr, phi, pi)
+        big0 = int(np.argmax(cat0[:, 1]))
+        i1 = int(np.nonzero(lab0 == big0)[0][0])
+        row = i1 - i1 % N
+        i2 = row + (i1 % N + 3) % N
+        phi = phi.copy()
+        phi[i1] = phi[i2] = 9.0
+        phi[row + (i1 % N + 1) % N] = phi[row + (i1 % N + 2) % N] = 8.0
+    else:
+        thr = 0.5 * O.volume_averages(m, phi, pi)[7]
+    cat, labels = O.islands(m, mode, thr, phi, pi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        rows, glab, _ = L.islands(mode, thr, labels=True)
+    sizes = np.array([r.nn for r in rows])
+    assert sizes.max() > 4096 and sizes.min() <= 4096
+    assert len(rows) == cat.shape[0]
+    assert np.array_equal(glab, labels)
+    for r, c in zip(rows, cat):
+        assert r.index == int(c[0]) and r.nn == int(c[1])
+        np.testing.assert_allclose([r.avg_phi, r.pot_en, r.kin_en, r.grad_en], c[5:9],
+                                   rtol=1e-12, atol=1e-14)
+        members = np.nonzero(labels == r.index)[0]
+        assert r.first_pos == members.min() + P
+        if r.peak_ties == 0:
+            assert r.peak_pos == int(c[3]) and r.peak_value == c[4]
+    _check_centroids(rows, glab, N)
+    if mode == 0:
+        big = rows[int(np.argmax(sizes))]
+        assert glab[i1] == big.index and glab[i2] == big.index
+        assert big.peak_ties == 1 and big.peak_pos == min(i1, i2) + P and big.peak_value == 9.0
+
+
 @pytest.mark.parametrize("kappa", [1.2247448713915889, 0.46775209018414249])
 @pytest.mark.parametrize("N", [16, 15, 64])
 def test_kappa_division_paths_bitwise(N, kappa):
