@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_transpose_xy(const T *__restrict__ in, 
 // each run in (hi, lo) form, and the run's last lane adds it to the wave's own
 // LDS histogram: no two lanes touch one bin at once and the order of the
 // additions is fixed by the launch.  The waves' histograms are merged in wave
-// order, the blocks' in block order by k_bin_final.  Counts are integers
+// order, the blocks' by k_bin_final in a fixed tree.  Counts are integers
 // (atomic adds are exact).
 template <typename T2>
 __global__ __launch_bounds__(256) void k_bin(const T2 *__restrict__ cphi, const T2 *__restrict__ cpi,
@@ -230,19 +230,32 @@ __global__ __launch_bounds__(256) void k_bin(const T2 *__restrict__ cphi, const 
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cpart[(long long)blockIdx.x * nbins + b] = sc[b];
 }
 
-__global__ void k_bin_final(const double *__restrict__ part, const int *__restrict__ cpart, int nb,
-                            int nbins, double *__restrict__ out, int *__restrict__ cout) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
+// one wave per shell: lane l merges the partials of blocks l, l + 64, ...,
+// then a fixed shuffle tree merges the lanes (the same order on every call)
+__global__ __launch_bounds__(256) void k_bin_final(const double *__restrict__ part,
+                                                   const int *__restrict__ cpart, int nb, int nbins,
+                                                   double *__restrict__ out, int *__restrict__ cout) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int b = (int)blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+  if (b >= nbins) return;  // whole waves
   double hi[3] = {0, 0, 0}, lo[3] = {0, 0, 0};
   int cnt = 0;
-  for (int k = 0; k < nb; k++) {
+  for (int k = lane; k < nb; k += 64) {
     const long long o = ((long long)k * nbins + b) * 6;
     for (int j = 0; j < 3; j++) two_merge(hi[j], lo[j], part[o + 2 * j], part[o + 2 * j + 1]);
     cnt += cpart[(long long)k * nbins + b];
   }
-  for (int j = 0; j < 3; j++) out[3 * b + j] = hi[j] + lo[j];
-  cout[b] = cnt;
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int j = 0; j < 3; j++) {
+      const double h2 = __shfl_down(hi[j], off, 64), l2 = __shfl_down(lo[j], off, 64);
+      two_merge(hi[j], lo[j], h2, l2);
+    }
+    cnt += __shfl_down(cnt, off, 64);
+  }
+  if (lane == 0) {
+    for (int j = 0; j < 3; j++) out[3 * b + j] = hi[j] + lo[j];
+    cout[b] = cnt;
+  }
 }
 
 // one field through transpose + r2c into dst
@@ -295,7 +308,7 @@ extern "C" int pn_spectrum_get(pn_ctx *c, pn_spectrum *out) {
                        (const float2 *)f->cphi, (const float2 *)f->cpi, f->tab, c->p.N, fac,
                        f->binsize, f->nbins, f->bin_part, f->cnt_part);
   PN_CHECK_LAUNCH(c);
-  hipLaunchKernelGGL(k_bin_final, dim3((f->nbins + 127) / 128), dim3(128), 0, c->stream,
+  hipLaunchKernelGGL(k_bin_final, dim3((f->nbins + 3) / 4), dim3(256), 0, c->stream,
                      f->bin_part, f->cnt_part, f->nblocks, f->nbins, f->bin_out, f->cnt_out);
   PN_CHECK_LAUNCH(c);
   std::vector<double> sums((size_t)f->nbins * 3);
