@@ -497,6 +497,32 @@ def test_island_centroids_brute_force():
     assert (rows[0].cx, rows[0].cy, rows[0].cz) == (15.5, 15.5, 9.0)
 
 
+@pytest.mark.parametrize("N,seed", [(32, 91), (48, 92), (64, 93)])
+def test_island_threshold_from_each_sides_averages(N, seed):
+    # the island threshold 2.57583 sqrt(<phi^2>) (main.cpp:70) taken from each
+    # side's own sums: the device's compensated tree and the reference's
+    # sequential loop may differ in the last bits, so the masks agree except
+    # at sites whose phi^2/2 + pi^2/2 (islands.cpp:76) lies between the two
+    # thresholds' squares over 2
+    m = O.model(N)
+    phi, pi = _fields(N, seed, phi0=0.0, amp=1.0)
+    thr_ref = 2.57583 * np.sqrt(O.volume_averages(m, phi, pi)[2])
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        thr_gpu = 2.57583 * np.sqrt(L.volume_averages().phi2)
+        rows, glab, _ = L.islands(0, thr_gpu, labels=True)
+    _, olab = O.islands(m, 0, thr_ref, phi, pi)
+    assert abs(thr_gpu - thr_ref) <= 1e-13 * thr_ref
+    assert len(rows) > 0
+    e = phi * phi * 0.5 + pi * pi * 0.5
+    diff = (glab >= 0) != (olab >= 0)
+    lo, hi = sorted((thr_gpu, thr_ref))
+    band = (e > lo * lo * 0.5 * (1 - 1e-14)) & (e <= hi * hi * 0.5 * (1 + 1e-14))
+    assert np.all(band[diff])
+    if thr_gpu == thr_ref:
+        assert np.array_equal(glab, olab)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_large_islands_match_bfs(mode):
     # a threshold low enough that one island spans most of a 40^3 lattice: its
