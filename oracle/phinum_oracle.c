@@ -26,6 +26,8 @@
  *   or_extrema               oracle/_ref bitwise; numpy argmax/argmin
  *   or_bin_table             brute-force count identities (sum = N^3)
  *   or_spectrum_bins         Parseval identity; single plane-wave mode
+ *   or_print_fp_rows         closed form on a single mode (omega = |B/A|, the
+ *                            1/((N a)^3 count) norm)
  *   or_islands_bfs           oracle/_ref bitwise; brute-force flood fill
  *   or_ic_modes              Hermitian symmetry; drand48 stream; spectrum
  *                            of the IC against the occupation formula

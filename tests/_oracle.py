@@ -170,6 +170,18 @@ def bin_table(N, a):
     return counts
 
 
+def print_fp_rows(N, a, counts, F, K):
+    """Print_fp's seven columns per shell and which shells it writes
+    (momentumspace.cpp:318-369)."""
+    nb = counts.size
+    rows = np.zeros(7 * nb)
+    valid = np.zeros(nb, dtype=np.int32)
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    lib().or_print_fp_rows(N, a, nb, iptr(c), dptr(np.ascontiguousarray(F, dtype=np.float64)),
+                           dptr(np.ascontiguousarray(K, dtype=np.float64)), dptr(rows), iptr(valid))
+    return rows.reshape(nb, 7), valid.astype(bool)
+
+
 def ref_modes(field_dense, N, a):
     """a^3-scaled r2c in the reference p-space order [pz][px][py] (clusterfft.cpp:185-225)."""
     f = field_dense.reshape(N, N, N)            # [z][y][x]

@@ -78,6 +78,20 @@ def test_driver_matches_oracle(tmp_path):
         np.testing.assert_allclose(row, ref, rtol=2e-5, atol=1e-9)
     dist = sorted(os.listdir(tmp_path / "output" / "1"))
     assert any(d.startswith("Distrib-1-") for d in dist)
+    # the spectrum file at tau = 0 (Print_fp, momentumspace.cpp:307-381) from
+    # the initial fields: the oracle's rows, to the 6 digits the file keeps
+    phi_ic, pi_ic = O.initial_conditions(m2, seed, phi0)
+    F, K, _ = O.spectrum(N, 0.5, phi_ic, pi_ic)
+    rows, valid = O.print_fp_rows(N, 0.5, O.bin_table(N, 0.5), F, K)
+    got = np.loadtxt(tmp_path / "output" / "1" / "Distrib-1-0.txt")
+    want = rows[valid]
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got[:, :2], want[:, :2], rtol=2e-6)  # p and the shell counts
+    # shells above the IC cutoff hold only the transforms' round-off (1e-32
+    # against 1e3, rocFFT and numpy's differently): compare the others
+    phys = want[:, 3] > 1e-20 * want[:, 3].max()
+    assert phys.sum() >= 4
+    np.testing.assert_allclose(got[phys], want[phys], rtol=2e-5)
 
 
 def _cont(m, phi, pi, start, stop):

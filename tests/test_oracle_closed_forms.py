@@ -123,6 +123,29 @@ def test_single_plane_wave_lands_in_its_shell(n):
     assert np.all(K == 0) and np.all(Q == 0)
 
 
+def test_print_fp_rows_on_a_single_mode():
+    # Print_fp (momentumspace.cpp:318-369) on phi = A cos(k.x), pi = B cos(k.x):
+    # one shell written, with omega = |B/A| and the norm 1/((N a)^3 count):
+    # FF norm = a^3 N^3 A^2 / (2 count), and the other columns from it
+    N, a, A, B = 16, 0.5, 0.7, -1.3
+    n = (2, 1, 0)
+    wave = _plane(N, a, n)
+    F, K, Q = O.spectrum(N, a, A * wave, B * wave)
+    counts = O.bin_table(N, a)
+    rows, valid = O.print_fp_rows(N, a, counts, F, K)
+    b = int(np.argmax(F))
+    # the other shells hold only the transform's round-off
+    others = valid.copy()
+    others[b] = False
+    assert valid[b] and np.all(rows[others, 3] <= 1e-20 * rows[b, 3])
+    binsize = (2.0 / a) * math.sin(math.pi / N)
+    ff = a ** 3 * N ** 3 * A * A / (2 * counts[b])
+    kk = a ** 3 * N ** 3 * B * B / (2 * counts[b])
+    want = [binsize * b, counts[b], abs(B / A), ff, kk, math.sqrt(ff * kk),
+            ff * (binsize * b) ** 3 / (2 * math.pi ** 2)]
+    np.testing.assert_allclose(rows[b], want, rtol=1e-12)
+
+
 def test_parseval_for_random_fields():
     N, a = 12, 0.5
     rng = np.random.default_rng(8)
