@@ -94,6 +94,30 @@ def test_driver_matches_oracle(tmp_path):
     np.testing.assert_allclose(got[phys], want[phys], rtol=2e-5)
 
 
+def test_driver_stops_when_the_false_vacuum_decays(tmp_path):
+    # false_vacuum_decayed (fieldanalysis.cpp:887-893) ends the run at the
+    # output where the average interaction turns negative (main.cpp:85-89).
+    # With g = 0.1 (kappa^2 = 3) the potential phi^2/2 - phi^4/24 +
+    # phi^6/(720 kappa^2) is negative around phi = 7.3: a run started there
+    # stops at its first output, after one SO row
+    N, seed = 16, 5
+    m = O.model(N, g=0.1)
+    phi0 = 7.3
+    args = [MAIN, "-runs", "1", "-Nt", str(N), "-at", "0.5", "-M", "1", "-Mmode", "1", "-L", "6",
+            "-Lmode", "-1", "-Kappa", "%.17g" % m["kappa"], "-T0", "0", "-TMax", "2.0", "-dt", "0.1",
+            "-TSO", "0.5", "-TMO", "1.0", "-Noise", "0.5", "-Ampl", "0", "-QS", "%.17g" % (np.sqrt(3.0) / 0.5),
+            "-Q2", "%.17g" % (np.sqrt(3.0) / 0.5), "-phi0_0", "%.17g" % phi0, "-seed", str(seed)]
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "false vacuum decayed at tau=0" in r.stdout
+    so = np.loadtxt(tmp_path / "output" / "1" / "SO-1.txt", ndmin=2)
+    assert so.shape[0] == 1
+    ophi, opi = O.initial_conditions(dict(m), seed, phi0)
+    avg = O.volume_averages(m, ophi, opi)
+    assert avg[8] < 0  # the interaction average the driver tested
+    np.testing.assert_allclose(so[0, 1:11], avg, rtol=2e-5, atol=1e-9)
+
+
 def _cont(m, phi, pi, start, stop):
     """oracle steps start..stop-1 with the factors those steps get in one run"""
     dtk, dtd, _ = O.time_factors(stop)
