@@ -278,6 +278,12 @@ const char *pn_tracker_last_error(const pn_tracker *t);
 int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
                       const char *outdir);
 int pn_tracker_count(const pn_tracker *t, int *n);
+/* The components of the last pn_tracker_update, as pn_islands numbers them
+ * (the islands[] of Full_Analysis's raster scan, islands.cpp:827-835),
+ * without running the analysis again: out->count receives their number and
+ * up to out->capacity entries are copied (PN_ERR_CAPACITY when it is
+ * smaller).  No labels. */
+int pn_tracker_catalog(const pn_tracker *t, pn_catalog *out);
 /* i-th tracked island in list order (after the update: the survivors in
  * decreasing volume of their previous entry, then the new ones). */
 int pn_tracker_get(const pn_tracker *t, int i, pn_tracked_island *out);

@@ -400,7 +400,7 @@ void Print_fp() {  // momentumspace.cpp:280-384
     double norm2 = (TotalBCounts[k] == 0 ? 0 : 1 / (double)(TotalBCounts[k]));
     double pabs = binsizeT * k;
     if (norm2 != 0 && TotalCorrF[k * Nc] != 0) {
-      if (k != 0) specOut << std::endl;
+      if (k != 0) specOut << '\n';
       specOut << pabs << " ";
       specOut << TotalBCounts[k] << " ";
       double FF = TotalCorrF[k] / Nc, KK = TotalCorrK[k] / Nc, QQ = TotalCorrQ[k] / Nc;
@@ -462,19 +462,15 @@ void Full_Analysis() {
     pn_tracker_get(tracker, i, &found_islands[i]);
     number_of_islands = std::max<int>(number_of_islands, (int)found_islands[i].index + 1);
   }
-  // the components of this call, as numbered by the raster scan
-  std::vector<pn_island> buf(4096);
-  pn_catalog cat{(int)buf.size(), 0, 0.0, buf.data(), nullptr};
-  rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
-  if (rc == PN_ERR_CAPACITY) {
-    buf.resize(cat.count);
-    cat.capacity = cat.count;
-    cat.islands = buf.data();
-    rc = pn_islands(LocalLattice::ctx, island_mode, thres, &cat);
-  }
-  check(rc, "Full_Analysis");
-  buf.resize(cat.count);
-  catalog = buf;
+  // the components of this call, as numbered by the raster scan (the
+  // tracker's update already computed them)
+  pn_catalog cat{0, 0, 0.0, nullptr, nullptr};
+  pn_tracker_catalog(tracker, &cat);  // the count
+  catalog.resize(cat.count);
+  cat.capacity = cat.count;
+  cat.islands = catalog.data();
+  if (pn_tracker_catalog(tracker, &cat) != PN_OK)
+    throw std::runtime_error(std::string("Full_Analysis: ") + pn_tracker_last_error(tracker));
 }
 
 // ---- checkpoint text (fieldanalysis.cpp:252-313) ---------------------------

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <new>
 #include <string>
@@ -39,6 +40,8 @@ struct pn_tracker {
   };
   std::vector<Isl> found;
   long long number_of_islands = 0;
+  std::vector<pn_island> cat;  // the last update's components (raster numbering)
+  int cap_hint = 1024;         // catalog room for the next update: the last count plus a quarter
   std::string err;
 };
 
@@ -61,6 +64,17 @@ extern "C" int pn_tracker_destroy(pn_tracker *t) {
 
 extern "C" const char *pn_tracker_last_error(const pn_tracker *t) {
   return t ? t->err.c_str() : pn_last_error(nullptr);
+}
+
+extern "C" int pn_tracker_catalog(const pn_tracker *t, pn_catalog *out) {
+  if (!t || !out || (out->capacity > 0 && !out->islands))
+    return tfail(const_cast<pn_tracker *>(t), PN_ERR_ARG, "pn_tracker_catalog: bad argument");
+  out->count = (int)t->cat.size();
+  const int n = std::min(out->count, out->capacity);
+  std::copy(t->cat.begin(), t->cat.begin() + n, out->islands);
+  if (out->count > out->capacity)
+    return tfail(const_cast<pn_tracker *>(t), PN_ERR_CAPACITY, "pn_tracker_catalog: capacity below the catalog");
+  return PN_OK;
 }
 
 extern "C" int pn_tracker_count(const pn_tracker *t, int *n) {
@@ -91,21 +105,27 @@ struct Coords {
   long long z(long long pos) const { return -1 + pos / (N * N); }
 };
 
+// The reference writes these rows with ostream << (default precision 6,
+// that is printf's %g, and plain integers) ending in std::endl.  Here each
+// row is formatted with %g into a buffer and written once, ending in '\n':
+// the same bytes, without the per-field stream formatting and the flush (a
+// write system call) per row.
 void write_entry(std::ofstream &o, const pn_track_entry &e, long long index, const Coords &c) {
-  o << e.at_time << ' ' << c.x(e.center_pos) << ' ' << c.y(e.center_pos) << ' ' << c.z(e.center_pos)
-    << "        ";
-  o << e.peak_value << ' ' << e.avg_phi << ' ' << e.volume << ' ';
-  o << e.grad_en << ' ' << e.pot_en << ' ' << e.kin_en << ' ' << index << std::endl;
+  char b[512];
+  const int n = std::snprintf(b, sizeof b, "%g %lld %lld %lld        %g %g %g %g %g %g %lld\n", e.at_time,
+                              c.x(e.center_pos), c.y(e.center_pos), c.z(e.center_pos), e.peak_value,
+                              e.avg_phi, e.volume, e.grad_en, e.pot_en, e.kin_en, index);
+  o.write(b, n);
 }
 
 // the islands.txt row (islands.cpp:683,717)
 void write_init_row(std::ofstream &o, const pn_tracker::Isl &s, int status, double thres) {
   const pn_track_entry &e0 = s.h[0];
-  o << std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.) << ' ' << e0.peak_value << ' '
-    << e0.at_time << ' ' << s.index << ' ' << status << ' '
-    << std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.) /
-           std::sqrt(std::log(e0.peak_value * e0.peak_value / thres / thres))
-    << std::endl;
+  const double r = std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.);
+  char b[256];
+  const int n = std::snprintf(b, sizeof b, "%g %g %g %lld %d %g\n", r, e0.peak_value, e0.at_time, s.index,
+                              status, r / std::sqrt(std::log(e0.peak_value * e0.peak_value / thres / thres)));
+  o.write(b, n);
 }
 
 }  // namespace
@@ -113,8 +133,10 @@ void write_init_row(std::ofstream &o, const pn_tracker::Isl &s, int status, doub
 extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
                                  const char *outdir) {
   if (!t || !c) return tfail(t, PN_ERR_ARG, "pn_tracker_update: NULL argument");
-  // the current components
-  std::vector<pn_island> cat(1024);
+  // the current components (room for the last count plus a quarter, so that
+  // the analysis normally runs once)
+  std::vector<pn_island> &cat = t->cat;
+  cat.resize((size_t)t->cap_hint);
   pn_catalog pc{(int)cat.size(), 0, 0.0, cat.data(), nullptr};
   int rc = pn_islands(c, mode, thr, &pc);
   if (rc == PN_ERR_CAPACITY) {
@@ -123,8 +145,12 @@ extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr,
     pc.islands = cat.data();
     rc = pn_islands(c, mode, thr, &pc);
   }
-  if (rc != PN_OK) return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
+  if (rc != PN_OK) {
+    cat.clear();
+    return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
+  }
   cat.resize(pc.count);
+  t->cap_hint = std::max(1024, pc.count + pc.count / 4);
 
   // largest latest volume first (islands.cpp:29-32,797); the comparison and
   // the algorithm are the reference's, so ties keep its order
@@ -163,7 +189,7 @@ extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr,
     if (files) {
       if (s.disp) {
         for (const pn_track_entry &e : s.h) write_entry(safe, e, s.index, co);
-        safe << std::endl;
+        safe << '\n';
       }
       write_init_row(init, s, s.status, thr);
     }
@@ -196,7 +222,7 @@ extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr,
         if (!bubbles.is_open()) bubbles.open(std::string(outdir) + "/expanding_bubbles.txt", std::ios::app);
         if (s.disp) write_init_row(init, s, 1, thr);
         for (size_t j = 1; j < s.h.size(); j++) write_entry(bubbles, s.h[j], s.index, co);
-        bubbles << std::endl;
+        bubbles << '\n';
       }
       s.status = 1;
     }
