@@ -146,6 +146,7 @@ def lib():
         "pn_tracker_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                                         C.c_char_p]),
         "pn_tracker_count": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "pn_tracker_catalog": (C.c_int, [C.c_void_p, P(PnCatalog)]),
         "pn_tracker_get": (C.c_int, [C.c_void_p, C.c_int, P(PnTrackedIsland)]),
         "pn_energy_density": (C.c_int, [C.c_void_p, dp]),
         "pn_field_diff_get": (C.c_int, [C.c_void_p, C.c_void_p, P(PnFieldDiff)]),
@@ -489,6 +490,18 @@ class Tracker:
                                       outdir.encode() if outdir else None)
         if rc != PN_OK:
             raise PhiNumError(rc, self.L.pn_tracker_last_error(self.h).decode())
+
+    def catalog(self):
+        """The components of the last update as pn_islands numbers them
+        (pn_tracker_catalog), a numpy structured array of PnIsland."""
+        cat = PnCatalog(0, 0, 0.0, None, None)
+        self.L.pn_tracker_catalog(self.h, C.byref(cat))  # the count
+        arr = (PnIsland * max(cat.count, 1))()
+        cat = PnCatalog(cat.count, 0, 0.0, arr, None)
+        rc = self.L.pn_tracker_catalog(self.h, C.byref(cat))
+        if rc != PN_OK:
+            raise PhiNumError(rc, self.L.pn_tracker_last_error(self.h).decode())
+        return np.ctypeslib.as_array(arr)[:cat.count]
 
     def islands(self):
         """[(index, status, disp, [(at_time, peak pos, peak value, avg phi, volume, PotEn,

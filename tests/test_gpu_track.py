@@ -81,6 +81,24 @@ def test_tracking_matches_oracle(mode, thr):
     _same_files(gfiles, wfiles)
 
 
+def test_tracker_catalog_is_the_update_components():
+    # pn_tracker_catalog returns the catalog the update computed, as
+    # pn_islands gives it for the same fields and threshold
+    N = 24
+    m = O.model(N)
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(N ** 3)
+    pi = 0.3 * rng.standard_normal(N ** 3)
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"]) as L:
+        L.set_fields(phi, pi)
+        T = pn.Tracker()
+        T.update(L, 0, 1.2, 0.0)
+        got = T.catalog()
+        want, _, _ = L.islands(0, 1.2, as_array=True)
+    assert len(got) == len(want) > 100
+    assert got.tobytes() == want.tobytes()
+
+
 def test_lookup_matches_catalog_labels():
     N = 12
     m = O.model(N)
