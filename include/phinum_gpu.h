@@ -214,6 +214,10 @@ int pn_evolve_timed(pn_ctx *c, int nsteps, const double *dtk, const double *dtd,
 int pn_volume_averages(pn_ctx *c, pn_averages *out);
 /* CalculateExtrema (fieldanalysis.cpp:319-397). */
 int pn_extrema_get(pn_ctx *c, pn_extrema *out);
+/* Both of the above from one pass over the fields, as Print_AveragesExtrema
+ * (fieldanalysis.cpp:748-799) calls them back to back; either output may be
+ * NULL.  The values are those of the two separate calls. */
+int pn_volume_analysis(pn_ctx *c, pn_averages *avg, pn_extrema *ext);
 /* nTBins of momentumspace.cpp:40-47 for this N and a. */
 int pn_spectrum_nbins(int N, double a);
 /* Calculate_CorrelationFunctions (momentumspace.cpp:140-278) with

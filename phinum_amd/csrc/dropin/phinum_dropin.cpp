@@ -257,11 +257,8 @@ void exchange() {}
 }  // namespace Communication
 
 // ---- fieldanalysis.cpp ----------------------------------------------------
-void CalculateVolumeAverages() {
+static void set_averages(const pn_averages &a) {
   using namespace LocalLattice;
-  pn_averages a;
-  flush_steps();
-  check(pn_volume_averages(ctx, &a), "CalculateVolumeAverages");
   phi[0] = avgphi[0] = a.phi;
   pi[0] = avgpi[0] = a.pi;
   phi2 = avgphi2 = a.phi2;
@@ -274,17 +271,28 @@ void CalculateVolumeAverages() {
   Charge = avgCharge = a.charge;
 }
 
-void CalculateExtrema() {
-  using namespace LocalLattice;
-  pn_extrema e;
+void CalculateVolumeAverages() {
+  pn_averages a;
   flush_steps();
-  check(pn_extrema_get(ctx, &e), "CalculateExtrema");
+  check(pn_volume_averages(LocalLattice::ctx, &a), "CalculateVolumeAverages");
+  set_averages(a);
+}
+
+static void set_extrema(const pn_extrema &e) {
+  using namespace LocalLattice;
   GlminPhi = minPhi = e.min;
   GlmaxPhi = maxPhi = e.max;
   GlmaxAbsPhi = maxAbsPhi = e.maxabs;
   minpos = (int)e.minpos;
   maxpos = (int)e.maxpos;
   maxAbspos = (int)e.maxabspos;
+}
+
+void CalculateExtrema() {
+  pn_extrema e;
+  flush_steps();
+  check(pn_extrema_get(LocalLattice::ctx, &e), "CalculateExtrema");
+  set_extrema(e);
 }
 
 static std::ofstream SpatialObs;
@@ -331,8 +339,14 @@ void Print_AveragesExtrema() {  // fieldanalysis.cpp:748-799
   } else if (tau >= numeric::LastSOoutput + TSO) {
     numeric::LastSOoutput += TSO;
   }
-  CalculateExtrema();
-  CalculateVolumeAverages();
+  // CalculateExtrema() and CalculateVolumeAverages() (fieldanalysis.cpp:765-766)
+  // from one pass over the fields
+  pn_averages a;
+  pn_extrema e;
+  flush_steps();
+  check(pn_volume_analysis(ctx, &a, &e), "Print_AveragesExtrema");
+  set_extrema(e);
+  set_averages(a);
   if (ID == 0 && write_output && SpatialObs.is_open()) {
     SpatialObs << numeric::LastSOoutput << ' ';
     for (int i = 0; i < Nc; i++) SpatialObs << avgphi[i] << ' ' << avgpi[i] << ' ';

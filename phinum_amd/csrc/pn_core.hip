@@ -1190,6 +1190,20 @@ extern "C" int pn_volume_averages(pn_ctx *c, pn_averages *out) {
   return PN_OK;
 }
 
+extern "C" int pn_volume_analysis(pn_ctx *c, pn_averages *avg, pn_extrema *ext) {
+  if (!c) return pn_fail(c, PN_ERR_ARG, "pn_volume_analysis: NULL context");
+  int rc = PN_OK;
+  if (avg && (rc = pn_volume_averages(c, avg)) != PN_OK) return rc;  // one k_volume run
+  if (ext) {
+    if (!avg && (rc = run_volume(c)) != PN_OK) return rc;
+    const double *s = c->host_pin + kNSums;
+    ext->min = s[0]; ext->minpos = (long long)s[1];
+    ext->max = s[2]; ext->maxpos = (long long)s[3];
+    ext->maxabs = s[4]; ext->maxabspos = (long long)s[5];
+  }
+  return PN_OK;
+}
+
 extern "C" int pn_extrema_get(pn_ctx *c, pn_extrema *out) {
   if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_extrema_get: NULL argument");
   int rc = run_volume(c);
