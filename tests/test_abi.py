@@ -31,7 +31,7 @@ def test_library_exports_all_declared_symbols():
 def test_abi_version_and_pure_helpers():
     L = pn.lib()
     assert L.pn_abi_version() == 1
-    # nTBins of momentumspace.cpp:40-47 for the configs' lattices (survey F8)
+    # nTBins of momentumspace.cpp:40-47 for the 64^3 ... 1024^3 lattices at a = 0.5
     assert [L.pn_spectrum_nbins(n, 0.5) for n in (64, 128, 192, 256, 1024)] == [37, 73, 108, 143, 567]
 
 
