@@ -135,6 +135,7 @@ def lib():
         "pn_evolve_timed": (C.c_int, [C.c_void_p, C.c_int, dp, dp, dp]),
         "pn_volume_averages": (C.c_int, [C.c_void_p, P(PnAverages)]),
         "pn_extrema_get": (C.c_int, [C.c_void_p, P(PnExtrema)]),
+        "pn_volume_analysis": (C.c_int, [C.c_void_p, P(PnAverages), P(PnExtrema)]),
         "pn_spectrum_nbins": (C.c_int, [C.c_int, C.c_double]),
         "pn_spectrum_get": (C.c_int, [C.c_void_p, P(PnSpectrum)]),
         "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),
@@ -310,6 +311,14 @@ class Lattice:
         out = PnExtrema()
         self._chk(self.L.pn_extrema_get(self.h, C.byref(out)))
         return out
+
+    def volume_analysis(self):
+        """(averages, extrema) from one pass over the fields, as
+        Print_AveragesExtrema calls the two back to back
+        (fieldanalysis.cpp:319-397,399-699)."""
+        avg, ext = PnAverages(), PnExtrema()
+        self._chk(self.L.pn_volume_analysis(self.h, C.byref(avg), C.byref(ext)))
+        return avg, ext
 
     def spectrum(self):
         nb = self.L.pn_spectrum_nbins(self.N, self.a)

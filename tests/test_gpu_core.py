@@ -383,6 +383,31 @@ def test_volume_averages_and_extrema(N, seed):
     assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
 
 
+@pytest.mark.parametrize("N,prec", [(16, pn.PN_FP64), (48, pn.PN_FP64), (32, pn.PN_FP32)])
+def test_volume_analysis_is_the_two_calls(N, prec):
+    # pn_volume_analysis: averages and extrema from one pass, the same bits as
+    # pn_volume_averages and pn_extrema_get called one after the other; with
+    # no averages argument it still fills the extrema
+    import ctypes as C
+    m = O.model(N)
+    phi, pi = _fields(N, 40 + N)
+    with _lat(m, prec) as L:
+        L.set_fields(phi, pi)
+        avg, ext = L.volume_analysis()
+        want_avg, want_ext = L.volume_averages(), L.extrema()
+        only = pn.PnExtrema()
+        L._chk(L.L.pn_volume_analysis(L.h, None, C.byref(only)))
+    assert np.array_equal(_bits(avg.as_array()), _bits(want_avg.as_array()))
+    for e in (ext, only):
+        assert (e.min, e.max, e.maxabs) == (want_ext.min, want_ext.max, want_ext.maxabs)
+        assert (e.minpos, e.maxpos, e.maxabspos) == (want_ext.minpos, want_ext.maxpos, want_ext.maxabspos)
+    if prec == pn.PN_FP64:
+        np.testing.assert_allclose(avg.as_array(), O.volume_averages(m, phi, pi), rtol=1e-13, atol=1e-15)
+        vals, idx = O.extrema(N, phi)
+        assert (ext.min, ext.max, ext.maxabs) == tuple(vals)
+        assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
+
+
 def test_extrema_ties_pick_first_site():
     N = 8
     m = O.model(N)
