@@ -55,10 +55,9 @@ struct pn_island_state {
   long long ccap = 0;                  // allocated chunks (large islands: half as many)
   int *chunk_isl = nullptr, *chunk_rank = nullptr, *big_isl = nullptr, *big_c0 = nullptr;
   double *cpart = nullptr;             // per chunk: (hi, lo) of the four sums
-  // pinned host staging of the per-island integers and sums, kept across calls
-  long long *h_ii = nullptr;
-  double *h_dd = nullptr;
-  size_t h_rows = 0;
+  // host staging of the per-island integers and sums, kept across calls
+  std::vector<long long> h_ii;
+  std::vector<double> h_dd;
 };
 
 void pn_free_islands(pn_ctx *c) {
@@ -84,8 +83,6 @@ void pn_free_islands(pn_ctx *c) {
   hipFree(s->big_isl);
   hipFree(s->big_c0);
   hipFree(s->cpart);
-  if (s->h_ii) (void)hipHostFree(s->h_ii);
-  if (s->h_dd) (void)hipHostFree(s->h_dd);
   delete s;
   c->isl = nullptr;
 }
@@ -793,24 +790,16 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
   if (dlab)
     hipLaunchKernelGGL(k_isl_labels, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, dlab);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess && (size_t)nisl > s->h_rows) {
-    // grow the staging by a quarter beyond this catalog; hipHostFree waits for the device
-    if (s->h_ii) (void)hipHostFree(s->h_ii);
-    if (s->h_dd) (void)hipHostFree(s->h_dd);
-    s->h_ii = nullptr;
-    s->h_dd = nullptr;
-    s->h_rows = 0;
-    const size_t rows = (size_t)nisl + (size_t)nisl / 4;
-    e = hipHostMalloc((void **)&s->h_ii, rows * kNI * sizeof(long long), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_dd, rows * kND * sizeof(double), hipHostMallocDefault);
-    if (e == hipSuccess) s->h_rows = rows;
+  if ((size_t)nisl * kNI > s->h_ii.size()) {
+    s->h_ii.resize((size_t)nisl * kNI + (size_t)nisl / 4 * kNI);
+    s->h_dd.resize((size_t)nisl * kND + (size_t)nisl / 4 * kND);
   }
-  const long long *ii = s->h_ii;
-  const double *dd = s->h_dd;
+  const long long *ii = s->h_ii.data();
+  const double *dd = s->h_dd.data();
   if (e == hipSuccess && nisl > 0)
-    e = hipMemcpyAsync(s->h_ii, s->iint, (size_t)nisl * kNI * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(s->h_ii.data(), s->iint, (size_t)nisl * kNI * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && nisl > 0)
-    e = hipMemcpyAsync(s->h_dd, s->idbl, (size_t)nisl * kND * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    e = hipMemcpyAsync(s->h_dd.data(), s->idbl, (size_t)nisl * kND * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && dlab)
     e = hipMemcpyAsync(out->labels, dlab, (size_t)n3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
