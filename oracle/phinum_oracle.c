@@ -22,15 +22,25 @@
  *                            homogeneous-field ODE (second order); time reversibility
  *   or_evolve_f32            within binary32 rounding of or_evolve over 20 steps;
  *                            a dropped phi^6 term or flipped phi^4 sign is not
- *   or_volume_averages       oracle/_ref bitwise; math.fsum closed forms
+ *   or_volume_averages       oracle/_ref bitwise; math.fsum closed forms; its
+ *                            Delta by sum Delta = vol (E + 1.5 (E + Charge - <pi^2>))
  *   or_extrema               oracle/_ref bitwise; numpy argmax/argmin
  *   or_bin_table             brute-force count identities (sum = N^3)
- *   or_spectrum_bins         Parseval identity; single plane-wave mode
+ *   or_spectrum_bins         Parseval identity; single plane-wave mode; every
+ *                            mode in the shell nearest |p|/pTmin (half a bin)
  *   or_print_fp_rows         closed form on a single mode (omega = |B/A|, the
  *                            1/((N a)^3 count) norm)
  *   or_islands_bfs           oracle/_ref bitwise; brute-force flood fill
  *   or_ic_modes              Hermitian symmetry; drand48 stream; spectrum
- *                            of the IC against the occupation formula
+ *                            of the IC against the occupation formula.
+ *                            PARITY UNPINNED: the draw order (phase1, phase2,
+ *                            |g1|, |g2|) and or_ensure_real's first-visited-wins
+ *                            and |F| sgn(Re F) rules -- FFTW is absent, the
+ *                            reference holds no fixtures, and no identity
+ *                            separates them from their alternatives
+ *   or_ic_fill               Parseval between modes and fields (the
+ *                            /(aN)^3 * sqrt((aN)^3) normalisation); field
+ *                            variances against the occupation closed form
  *   or_tlm_(...), or_adj_(...) dot-product test (<TLM v,w> = <v,ADJ w>);
  *                            central finite differences
  */
