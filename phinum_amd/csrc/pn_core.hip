@@ -498,6 +498,12 @@ int pn_reduce_blocks(const pn_ctx *c) { return c->red_blocks; }
 
 static constexpr int kRedThreads = 256;
 static constexpr int kNSums = 9;   // phi, pi, phi2, pi2, phipi, phi4, phi6, grad, inter
+// block partials of the volume sums per call (and their grid's size cap), and
+// tiles per call (N <= 2048 has at most 8 x 2048 columns of tiles)
+static constexpr int kVolMaxBlocks = 2048;
+static constexpr int kVolMaxTiles = 16384;
+// devices a process may use with the per-device caches below
+static constexpr int kScratchMaxDevices = 64;
 
 extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
   if (!out) return pn_fail(nullptr, PN_ERR_ARG, "pn_create: out is NULL");
@@ -529,7 +535,7 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
   c->k.c2 = p->mass_mode * p->m2;
   c->k.lmode = (double)p->lambda_mode;
   c->k.kappa = p->kappa;
-  c->red_blocks = (int)std::min<long long>(1024, (c->n3 + kRedThreads - 1) / kRedThreads);
+  c->red_blocks = kVolMaxBlocks;
 
   auto cleanup = [&](int code) {
     pn_destroy(c);
@@ -555,6 +561,7 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
   }
   if (hipMalloc(&c->red_part, (size_t)c->red_blocks * 2 * kNSums * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->ext_part, (size_t)c->red_blocks * 6 * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->tile_flag, (size_t)(kVolMaxTiles + kVolMaxBlocks) * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->red_out, 64 * sizeof(double)) != hipSuccess)
     return cleanup(pn_fail(nullptr, PN_ERR_NOMEM, "pn_create: cannot allocate reduction scratch"));
   if (hipHostMalloc(&c->host_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
@@ -583,7 +590,7 @@ int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, 
   c->k.c2 = p->mass_mode * p->m2;
   c->k.lmode = (double)p->lambda_mode;
   c->k.kappa = p->kappa;
-  c->red_blocks = (int)std::min<long long>(1024, (c->n3 + kRedThreads - 1) / kRedThreads);
+  c->red_blocks = kVolMaxBlocks;
   c->phi[0] = phi0;
   c->phi[1] = phi1;
   c->pi = pi;
@@ -591,6 +598,7 @@ int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, 
   c->stream = s;
   if (hipMalloc(&c->red_part, (size_t)c->red_blocks * 2 * kNSums * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->ext_part, (size_t)c->red_blocks * 6 * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&c->tile_flag, (size_t)(kVolMaxTiles + kVolMaxBlocks) * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->red_out, 64 * sizeof(double)) != hipSuccess ||
       hipHostMalloc(&c->host_pin, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     pn_destroy(c);
@@ -628,6 +636,7 @@ extern "C" int pn_destroy(pn_ctx *c) {
   }
   if (c->red_part) (void)hipFree(c->red_part);
   if (c->ext_part) (void)hipFree(c->ext_part);
+  if (c->tile_flag) (void)hipFree(c->tile_flag);
   if (c->red_out) (void)hipFree(c->red_out);
   if (c->host_pin) (void)hipHostFree(c->host_pin);
   if (c->stream && !c->is_view) (void)hipStreamDestroy(c->stream);
@@ -1005,90 +1014,313 @@ __device__ __forceinline__ void ext_merge(Ext &a, const Ext &b) {
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ long long shfl_ll(long long v, int src) { return __shfl(v, src, 64); }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_volume(const T *__restrict__ phi, const T *__restrict__ pi,
-                                                int N, RedCoef k, double *__restrict__ part,
-                                                long long *__restrict__ epart) {
-  double hi[kNSums], lo[kNSums];
+// The divisions of the potential sum (fieldanalysis.cpp:571) are by run
+// constants, 24 and 720*Nc*Nc*Kappa*Kappa: two-term reciprocals
+// (pn_fastdiv.h) give the IEEE quotient in two or four FMA-pipe operations.
+struct VolCoef {
+  RedCoef r;
+  pn_recip r24, rd6;
+  int fdiv;    // both divisors lie where pn_div_fast is exact
+  int d6_bmr;  // the division by d6 needs no correction step
+};
+
+// Tiles of RB rows x XC vectors of W consecutive x sites, ZB planes deep; a
+// block of 256 threads takes one tile at a time and streams its planes in z,
+// so the z neighbour of plane z is the own field of plane z+1.
+struct VolShape {
+  int W, NV, XC, RB, nxc, nyt, nzb, ZB, tiles;
+};
+// The main pass runs as many blocks as are resident at once, each taking tiles
+// in turn; the z blocks are cut to balance the blocks' work (kVolTileStart:
+// a tile's start-up cost in planes).
+constexpr int kVolTileStart = 3;
+// planes of a tile held in registers at once (the loads' look-ahead)
+constexpr int kVolRing = 4;
+
+static VolShape vol_shape(int N, int W, int grid) {
+  VolShape s;
+  s.W = W;
+  s.NV = N / W;
+  s.XC = s.NV < kRedThreads ? s.NV : kRedThreads;
+  s.nxc = (s.NV + kRedThreads - 1) / kRedThreads;
+  s.RB = std::min(kRedThreads / s.XC, N);
+  s.nyt = (N + s.RB - 1) / s.RB;
+  const long long cols = (long long)s.nxc * s.nyt;  // at most kVolMaxTiles for N <= 2048
+  // planes per tile: the busiest block's planes, plus a few planes' worth of
+  // start-up per tile, as small as it gets (z blocks of at least 4 planes)
+  long long best = -1, best_nzb = 1;
+  const long long nz_max = std::max<long long>(1, std::min<long long>(N / 4, kVolMaxTiles / cols));
+  for (long long nzb = 1; nzb <= nz_max; nzb++) {
+    const long long zb = (N + nzb - 1) / nzb;
+    if ((N + zb - 1) / zb != nzb) continue;  // the same z blocks as a smaller count
+    const long long per_block = (cols * nzb + grid - 1) / grid;
+    const long long cost = per_block * (zb + kVolTileStart);
+    if (best < 0 || cost < best) { best = cost; best_nzb = nzb; }
+  }
+  s.ZB = (int)((N + best_nzb - 1) / best_nzb);
+  s.nzb = (N + s.ZB - 1) / s.ZB;
+  s.tiles = (int)(cols * s.nzb);
+  return s;
+}
+
+// Device scratch of one volume analysis: the main pass's block partials
+// (kVolMaxBlocks * 2 * kNSums doubles and kVolMaxBlocks * 6 extrema words) and
+// its tile flags (kVolMaxTiles).
+struct VolScratch {
+  double *part;
+  long long *epart;
+  int *tflag;
+  int *bflag;  // per block: one of its tiles has a flag (kVolMaxBlocks)
+};
+
+// One vector's terms (fieldanalysis.cpp:331-355,437-571), sites in raster
+// order.  The sums of phi, pi and phi*pi, whose terms can have either sign,
+// take every term through an exact two-sum; so does the interaction, with the
+// two terms of a site (:457 and :571) added first.  The terms of phi^2, pi^2,
+// phi^4, phi^6 and the gradient are never negative: a run of the terms of two
+// planes' vectors (at most 2W) is added first (relative error below 2W ulp
+// of the run, which is at most the total) and the run takes one two-sum.
+// FAST: the wave's sites all have |phi| in [2^-100, 2^100), so
+//  * phi^4 and phi^6 lie where the two-term divisions are exact;
+//  * phi*phi is a normal number, for which sqrt(RN(phi*phi)) == |phi| in
+//    binary arithmetic (S. Boldo, "Stupid is as stupid does: taking the
+//    square root of the square of a floating-point number", 2015): MaxAbs
+//    takes |phi| without the square root;
+//  * every nonzero difference of neighbours is at least 2^-152, so with a2 a
+//    power of two (1/a2 within [2^-200, 2^200]) no product or sum of the
+//    gradient under- or overflows, and ((dx^2/a2 + dy^2/a2) + dz^2/a2) equals
+//    ((dx^2 + dy^2) + dz^2) * (1/a2) bit for bit (scaling by a power of two
+//    commutes with rounding).
+template <bool POW2, bool FAST>
+__device__ __forceinline__ void vol_site(const VolCoef &k, double f, double p, double fx, double fy,
+                                         double fz, long long rp, bool first, double (&r)[5],
+                                         double (&hi)[kNSums], double (&lo)[kNSums], Ext &e) {
+  const double q = f * f;  // the reference's 0.0 + phi*phi: never -0
+  two_acc(hi[0], lo[0], f);
+  two_acc(hi[1], lo[1], p);
+  two_acc(hi[4], lo[4], f * p);
+  const double q2 = q * q, q3 = q2 * q;
+  double g, v, ab;
+  if (FAST) {
+    const double dx = fx - f, dy = fy - f, dz = fz - f;
+    g = POW2 ? ((dx * dx + dy * dy) + dz * dz) * k.r.inv_a2
+             : (dx * dx / k.r.a2 + dy * dy / k.r.a2) + dz * dz / k.r.a2;  // :445
+    double v6 = pn_div_bmr(q3, k.rd6);
+    if (!k.d6_bmr) v6 = pn_div_correct(q3, k.rd6, v6);
+    v = k.r.lmode * pn_div_bmr(q2, k.r24) + v6;  // :571
+    ab = fabs(f);
+  } else {
+    const double dx = fx - f, dy = fy - f, dz = fz - f;
+    const double gx = POW2 ? dx * dx * k.r.inv_a2 : dx * dx / k.r.a2;
+    const double gy = POW2 ? dy * dy * k.r.inv_a2 : dy * dy / k.r.a2;
+    const double gz = POW2 ? dz * dz * k.r.inv_a2 : dz * dz / k.r.a2;
+    g = (gx + gy) + gz;
+    v = k.r.lmode * (q2 / 24.0) + q3 / k.r.d6;
+    ab = sqrt(0.0 + f * f);
+  }
+  two_acc(hi[8], lo[8], k.r.cm * q + v);  // :457, :571
+  const double t5[5] = {q, p * p, q2, q3, g};
 #pragma unroll
-  for (int s = 0; s < kNSums; s++) { hi[s] = 0.0; lo[s] = 0.0; }
-  Ext e;
-  e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;  // fieldanalysis.cpp:324
-  e.imn = e.imx = e.iab = -1;
-  // rows (y, z) walked by the grid; a block takes blockDim/N rows at a time
-  // (one when N >= blockDim), x from the thread index: no per-site division
+  for (int u = 0; u < 5; u++) r[u] = first ? t5[u] : r[u] + t5[u];
+  // a thread visits its sites in raster order, so strict comparisons keep
+  // the first index
+  if (ab > e.ab) { e.ab = ab; e.iab = rp; }
+  if (f > e.mx) { e.mx = f; e.imx = rp; }
+  if (f < e.mn) { e.mn = f; e.imn = rp; }
+}
+
+__device__ __forceinline__ void vol_runs(const double (&r)[5], double (&hi)[kNSums],
+                                         double (&lo)[kNSums]) {
+  two_acc(hi[2], lo[2], r[0]);
+  two_acc(hi[3], lo[3], r[1]);
+  two_acc(hi[5], lo[5], r[2]);
+  two_acc(hi[6], lo[6], r[3]);
+  two_acc(hi[7], lo[7], r[4]);
+}
+
+// a vector of W sites; start: the vector begins the runs of the never
+// negative terms (vol_runs adds them to the sums)
+template <int W, bool POW2, bool FAST, typename V>
+__device__ __forceinline__ void vol_vec(const VolCoef &k, const V &fv, const V &pv, const V &fy,
+                                        const V &fz, double fxw, long long rp, bool start,
+                                        double (&r)[5], double (&hi)[kNSums],
+                                        double (&lo)[kNSums], Ext &e) {
+#pragma unroll
+  for (int j = 0; j < W; j++)
+    vol_site<POW2, FAST>(k, (double)fv[j], (double)pv[j],
+                         (j + 1 < W) ? (double)fv[(j + 1) % W] : fxw, (double)fy[j],
+                         (double)fz[j], rp + j, start && j == 0, r, hi, lo, e);
+}
+
+// One plane of a tile's vector: through the fast form when the whole wave
+// allows it (PASS 0) or through the IEEE form when it does not (PASS 1).
+// Returns 1 when the vector was summed, 2 when PASS 0 skipped it.
+template <int W, bool POW2, int PASS, bool FASTOK, typename V>
+__device__ __forceinline__ int vol_plane(const VolCoef &k, const V &f, const V &p, const V &fy,
+                                         const V &fz, double fxw, long long rp, bool start,
+                                         double (&r)[5], double (&hi)[kNSums],
+                                         double (&lo)[kNSums], Ext &e) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < W; j++) ok = ok && fast_div_site((double)f[j]);
+  const bool fast = FASTOK && k.fdiv && __all(ok);
+  if (PASS == 0 && !FASTOK) {
+    vol_vec<W, POW2, false>(k, f, p, fy, fz, fxw, rp, start, r, hi, lo, e);
+    return 1;
+  }
+  if (PASS == 0) {
+    if (!fast) return 2;
+    vol_vec<W, POW2, true>(k, f, p, fy, fz, fxw, rp, start, r, hi, lo, e);
+    return 1;
+  }
+  if (fast) return 0;
+  vol_vec<W, POW2, false>(k, f, p, fy, fz, fxw, rp, start, r, hi, lo, e);
+  return 1;
+}
+
+// One tile of the volume sums.  PASS 0 (k_volume): a vector whose wave has
+// every site in the fast range is summed, any other is skipped and the call
+// returns true (FASTOK false, for divisors outside the fast form's range: every
+// vector is summed with the IEEE divisions and none is skipped).  PASS 1 (k_volume_final, over the tiles PASS 0 flagged): only
+// the skipped vectors are summed, with the IEEE divisions and square root.
+// Both passes map threads to sites and group them into waves the same way, so
+// every vector is summed by exactly one of them.  Keeping the IEEE sequences
+// out of the main kernel keeps its register count, and its occupancy, that
+// of the fast form.
+template <typename T, int W, bool POW2, int PASS, bool FASTOK = true>
+__device__ __forceinline__ bool vol_tile(const T *__restrict__ phi, const T *__restrict__ pi,
+                                         int N, const VolShape &s, const VolCoef &k, int t, int lt,
+                                         double (&hi)[kNSums], double (&lo)[kNSums], Ext &e) {
+  typedef typename VecT<T, W>::type V;
   const long long P = (long long)N * N;
-  const int rows = N * N;
-  const int tpr = N < (int)blockDim.x ? N : (int)blockDim.x;
-  const int rpi = (int)blockDim.x / tpr;
-  const int tr = (int)threadIdx.x / tpr, tx = (int)threadIdx.x % tpr;
-  for (int r0 = (int)blockIdx.x * rpi; r0 < rows; r0 += (int)gridDim.x * rpi) {
-    const int r = r0 + tr;
-    if (tr >= rpi || r >= rows) continue;
-    const int y = r % N, z = r / N;
-    const long long rowi = (long long)r * N;
-    const long long dyp = (long long)((y + 1 == N ? 0 : y + 1) - y) * N;
-    const long long dzp = (long long)((z + 1 == N ? 0 : z + 1) - z) * P;
-    for (int x = tx; x < N; x += tpr) {
-      const long long i = rowi + x;
-      const long long ixp = rowi + (x + 1 == N ? 0 : x + 1);
-      const double f = (double)phi[i], p = (double)pi[i];
-      const double fx = (double)phi[ixp], fy = (double)phi[i + dyp], fz = (double)phi[i + dzp];
-      double q = 0.0;
-      q += f * f;
-      two_acc(hi[0], lo[0], f);
-      two_acc(hi[1], lo[1], p);
-      two_acc(hi[2], lo[2], f * f);
-      two_acc(hi[3], lo[3], p * p);
-      two_acc(hi[4], lo[4], f * p);
-      two_acc(hi[5], lo[5], q * q);
-      two_acc(hi[6], lo[6], (q * q) * q);
-      const double dx = fx - f, dy = fy - f, dz = fz - f;
-      // :445; with a2 a power of two the division is the exact product
-      const double gx = k.pow2 ? dx * dx * k.inv_a2 : dx * dx / k.a2;
-      const double gy = k.pow2 ? dy * dy * k.inv_a2 : dy * dy / k.a2;
-      const double gz = k.pow2 ? dz * dz * k.inv_a2 : dz * dz / k.a2;
-      two_acc(hi[7], lo[7], (gx + gy) + gz);
-      two_acc(hi[8], lo[8], k.cm * (f * f));                                        // :457
-      two_acc(hi[8], lo[8], k.lmode * ((q * q) / 24.0) + ((q * q) * q) / k.d6);     // :571
-      // extrema (fieldanalysis.cpp:331-355); AbsPhi = sqrt(0 + phi*phi); a
-      // thread visits its sites in raster order, so strict comparisons keep
-      // the first index
-      const long long rp = i + P;
-      const double ab = sqrt(0.0 + f * f);
-      if (ab > e.ab) { e.ab = ab; e.iab = rp; }
-      if (f > e.mx) { e.mx = f; e.imx = rp; }
-      if (f < e.mn) { e.mn = f; e.imn = rp; }
+  const int tr = lt / s.XC, tx = lt - tr * s.XC;  // lt: the thread's index in its tile group
+  // tile = (z block, row block, x chunk), x chunk fastest
+  const int xc = t % s.nxc;
+  t /= s.nxc;
+  const int yt = t % s.nyt, zb = t / s.nyt;
+  const int xv = xc * kRedThreads + tx, y = yt * s.RB + tr;
+  if (tr >= s.RB || xv >= s.NV || y >= N) return false;
+  const int z0 = zb * s.ZB, z1 = min(z0 + s.ZB, N);
+  const int x = xv * W;
+  const long long row = (long long)y * N;
+  const long long dyp = (long long)((y + 1 == N ? 0 : y + 1) - y) * N;
+  const long long xw = row + (x + W == N ? 0 : x + W);  // x neighbour of the last site
+  bool skipped = false;
+  // A ring of kVolRing planes in registers: slot u holds plane z of the tile
+  // with z - z0 = u (mod kVolRing): its field, momentum, y+1 row and x
+  // neighbour.  A slot is refilled with the plane kVolRing further on right
+  // after its sums, so each plane's loads are in flight while the planes
+  // before it are summed.  Plane z1 (periodic: plane 0 after plane N-1) holds
+  // the z neighbour of the tile's last plane; refills past it load plane z1
+  // again (cached), so every iteration issues the same loads and the waits
+  // for them stay exact.
+  V F[kVolRing], Pm[kVolRing], Y[kVolRing];
+  double X[kVolRing];
+  auto load = [&](int u, int zz) {
+    zz = zz < z1 ? zz : z1;
+    const long long iz = (long long)(zz == N ? 0 : zz) * P + row;
+    F[u] = *(const V *)&phi[iz + x];
+    Pm[u] = __builtin_nontemporal_load((const V *)&pi[iz + x]);
+    Y[u] = *(const V *)&phi[iz + dyp + x];
+    X[u] = (double)phi[iz - row + xw];
+  };
+#pragma unroll
+  for (int u = 0; u < kVolRing; u++) load(u, z0 + u);
+  for (int zg = z0; zg < z1; zg += kVolRing) {
+    double r[5];
+    bool started = false;
+#pragma unroll
+    for (int u = 0; u < kVolRing; u++) {
+      const int z = zg + u;
+      if (z < z1) {
+        const int res = vol_plane<W, POW2, PASS, FASTOK>(k, F[u], Pm[u], Y[u], F[(u + 1) % kVolRing],
+                                                         X[u], (long long)z * P + row + x + P,
+                                                         !started, r, hi, lo, e);
+        started = started || (res & 1);
+        skipped = skipped || (res & 2);
+        // runs of two planes' terms at most
+        if ((u & 1) || z + 1 == z1) {
+          if (started) vol_runs(r, hi, lo);
+          started = false;
+        }
+      }
+      load(u, z + kVolRing);
     }
   }
-  // wavefront reduction
-  for (int off = 32; off > 0; off >>= 1) {
+  return skipped;
+}
+
+__device__ __forceinline__ void vol_init(double (&hi)[kNSums], double (&lo)[kNSums], Ext &e) {
 #pragma unroll
-    for (int s = 0; s < kNSums; s++) {
-      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
-      two_merge(hi[s], lo[s], h2, l2);
+  for (int q = 0; q < kNSums; q++) { hi[q] = 0.0; lo[q] = 0.0; }
+  e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;  // fieldanalysis.cpp:324
+  e.imn = e.imx = e.iab = -1;
+}
+
+// merges the block's threads in a fixed order (wave trees, then the waves in
+// order) into sh[0] and se[0]
+template <int NT = kRedThreads>
+__device__ __forceinline__ void vol_block_merge(double (&hi)[kNSums], double (&lo)[kNSums], Ext &e,
+                                                double (*sh)[2 * kNSums], Ext *se) {
+  // one sum at a time through the shuffle tree (fewer registers live than
+  // all sums level by level)
+#pragma unroll
+  for (int q = 0; q < kNSums; q++)
+    for (int off = 32; off > 0; off >>= 1) {
+      const double h2 = __shfl_down(hi[q], off, 64), l2 = __shfl_down(lo[q], off, 64);
+      two_merge(hi[q], lo[q], h2, l2);
     }
+  for (int off = 32; off > 0; off >>= 1) {
     Ext o;
     o.mn = __shfl_down(e.mn, off, 64); o.imn = __shfl_down(e.imn, off, 64);
     o.mx = __shfl_down(e.mx, off, 64); o.imx = __shfl_down(e.imx, off, 64);
     o.ab = __shfl_down(e.ab, off, 64); o.iab = __shfl_down(e.iab, off, 64);
     ext_merge(e, o);
   }
-  __shared__ double sh[4][2 * kNSums];
-  __shared__ Ext se[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
 #pragma unroll
-    for (int s = 0; s < kNSums; s++) { sh[w][2 * s] = hi[s]; sh[w][2 * s + 1] = lo[s]; }
+    for (int q = 0; q < kNSums; q++) { sh[w][2 * q] = hi[q]; sh[w][2 * q + 1] = lo[q]; }
     se[w] = e;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int ww = 1; ww < (int)(blockDim.x >> 6); ww++) {
-      for (int s = 0; s < kNSums; s++) two_merge(sh[0][2 * s], sh[0][2 * s + 1], sh[ww][2 * s], sh[ww][2 * s + 1]);
+  if (threadIdx.x == 0)
+    for (int ww = 1; ww < NT / 64; ww++) {
+      for (int q = 0; q < kNSums; q++) two_merge(sh[0][2 * q], sh[0][2 * q + 1], sh[ww][2 * q], sh[ww][2 * q + 1]);
       ext_merge(se[0], se[ww]);
     }
-    for (int s = 0; s < 2 * kNSums; s++) part[(long long)blockIdx.x * 2 * kNSums + s] = sh[0][s];
+}
+
+// Main pass: a grid of as many blocks as are resident at once, each taking
+// tiles slot, slot + gridDim, ... (in each round of gridDim tiles, XCD k takes
+// a contiguous run: tiles that share halo rows); per block one partial, per
+// tile a flag saying whether it skipped vectors for the final pass.
+template <typename T, int W, bool POW2, bool FASTOK>
+__global__ __launch_bounds__(kRedThreads) void k_volume(const T *__restrict__ phi,
+                                                        const T *__restrict__ pi, int N,
+                                                        VolShape s, VolCoef k,
+                                                        double *__restrict__ part,
+                                                        long long *__restrict__ epart,
+                                                        int *__restrict__ tflag,
+                                                        int *__restrict__ bflag) {
+  double hi[kNSums], lo[kNSums];
+  Ext e;
+  vol_init(hi, lo, e);
+  const int slot = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int block_any = 0;
+  for (int base = 0; base + slot < s.tiles; base += (int)gridDim.x) {
+    const bool skipped =
+        vol_tile<T, W, POW2, 0, FASTOK>(phi, pi, N, s, k, base + slot, (int)threadIdx.x, hi, lo, e);
+    const int any = __syncthreads_or(skipped);
+    if (threadIdx.x == 0) tflag[base + slot] = any;
+    block_any |= any;
+  }
+  if (threadIdx.x == 0) bflag[blockIdx.x] = block_any;
+  __shared__ double sh[kRedThreads / 64][2 * kNSums];
+  __shared__ Ext se[kRedThreads / 64];
+  vol_block_merge(hi, lo, e, sh, se);
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 2 * kNSums; q++) part[(long long)blockIdx.x * 2 * kNSums + q] = sh[0][q];
     long long *ep = epart + (long long)blockIdx.x * 6;
     ep[0] = __double_as_longlong(se[0].mn); ep[1] = se[0].imn;
     ep[2] = __double_as_longlong(se[0].mx); ep[3] = se[0].imx;
@@ -1096,111 +1328,206 @@ __global__ __launch_bounds__(256) void k_volume(const T *__restrict__ phi, const
   }
 }
 
-// single wave merges the block partials in a fixed order
-__global__ __launch_bounds__(64) void k_volume_final(const double *__restrict__ part,
-                                                     const long long *__restrict__ epart, int nb,
-                                                     double *__restrict__ out) {
+// Values written by k_volume_final: the ten averages in pn_averages order,
+// then min, minpos, max, maxpos, maxabs, maxabspos (positions as doubles).
+constexpr int kVolOut = 16;
+
+// One block: the vectors the main pass skipped (none in the usual case;
+// flagged tiles in order), then the main pass's block partials (thread t takes
+// partials t, t + kVolFinalThreads, ...), merged in a fixed order (wave trees,
+// then the waves in order); then the averages as fieldanalysis.cpp:619-638
+// forms them (one rank: /nodes == /1).  One tile group of threads, so the
+// IEEE path keeps its registers without spilling.
+constexpr int kVolFinalThreads = kRedThreads;
+template <typename T, int W, bool POW2>
+__global__ __launch_bounds__(kVolFinalThreads) void k_volume_final(
+    const T *__restrict__ phi, const T *__restrict__ pi, int N, VolShape s, VolCoef k,
+    const double *__restrict__ part, const long long *__restrict__ epart, int nb,
+    const int *__restrict__ tflag, const int *__restrict__ bflag, double vol,
+    double *__restrict__ out) {
   double hi[kNSums], lo[kNSums];
-#pragma unroll
-  for (int s = 0; s < kNSums; s++) { hi[s] = 0.0; lo[s] = 0.0; }
   Ext e;
-  e.mn = 10000000.0; e.mx = -10000000.0; e.ab = 0.0;
-  e.imn = e.imx = e.iab = -1;
-  for (int b = threadIdx.x; b < nb; b += 64) {
-    for (int s = 0; s < kNSums; s++)
-      two_merge(hi[s], lo[s], part[(long long)b * 2 * kNSums + 2 * s], part[(long long)b * 2 * kNSums + 2 * s + 1]);
+  vol_init(hi, lo, e);
+  int mine = 0;
+  for (int b = threadIdx.x; b < nb; b += kVolFinalThreads) mine |= bflag[b];
+  if (__syncthreads_or(mine))
+    for (int t = 0; t < s.tiles; t++)
+      if (tflag[t]) vol_tile<T, W, POW2, 1>(phi, pi, N, s, k, t, (int)threadIdx.x, hi, lo, e);
+  for (int b = threadIdx.x; b < nb; b += kVolFinalThreads) {
+    const double *pb = part + (long long)b * 2 * kNSums;
+    double v[2 * kNSums];
+#pragma unroll
+    for (int q = 0; q < 2 * kNSums; q++) v[q] = pb[q];
     const long long *ep = epart + (long long)b * 6;
     Ext o;
     o.mn = __longlong_as_double(ep[0]); o.imn = ep[1];
     o.mx = __longlong_as_double(ep[2]); o.imx = ep[3];
     o.ab = __longlong_as_double(ep[4]); o.iab = ep[5];
+#pragma unroll
+    for (int q = 0; q < kNSums; q++) two_merge(hi[q], lo[q], v[2 * q], v[2 * q + 1]);
     ext_merge(e, o);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    for (int s = 0; s < kNSums; s++) {
-      const double h2 = __shfl_down(hi[s], off, 64), l2 = __shfl_down(lo[s], off, 64);
-      two_merge(hi[s], lo[s], h2, l2);
-    }
-    Ext o;
-    o.mn = __shfl_down(e.mn, off, 64); o.imn = __shfl_down(e.imn, off, 64);
-    o.mx = __shfl_down(e.mx, off, 64); o.imx = __shfl_down(e.imx, off, 64);
-    o.ab = __shfl_down(e.ab, off, 64); o.iab = __shfl_down(e.iab, off, 64);
-    ext_merge(e, o);
-  }
+  __shared__ double sh[kVolFinalThreads / 64][2 * kNSums];
+  __shared__ Ext se[kVolFinalThreads / 64];
+  vol_block_merge<kVolFinalThreads>(hi, lo, e, sh, se);
   if (threadIdx.x == 0) {
-    for (int s = 0; s < kNSums; s++) out[s] = hi[s] + lo[s];
-    out[kNSums + 0] = e.mn; out[kNSums + 1] = (double)e.imn;
-    out[kNSums + 2] = e.mx; out[kNSums + 3] = (double)e.imx;
-    out[kNSums + 4] = e.ab; out[kNSums + 5] = (double)e.iab;
+    e = se[0];
+    double S[kNSums];
+    for (int q = 0; q < kNSums; q++) S[q] = sh[0][2 * q] + sh[0][2 * q + 1];
+    double phi1 = S[0], pi1 = S[1], phi2 = S[2], pi2 = S[3], phipi = S[4], phi4 = S[5], phi6 = S[6];
+    double G = S[7], I = S[8];
+    phi1 /= (vol); pi1 /= (vol);
+    double Charge = (0.5 * pi2 - G / 6. - I);
+    double Energy = (0.5 * pi2 + 0.5 * G + I);
+    Energy /= (vol); I /= (vol); Charge /= (vol);
+    phi2 /= (vol); pi2 /= (vol); phi4 /= (vol); phi6 /= (vol); phipi /= (vol);
+    out[0] = phi1; out[1] = pi1; out[2] = phi2; out[3] = pi2; out[4] = phipi;
+    out[5] = phi4; out[6] = phi6; out[7] = Energy; out[8] = I; out[9] = Charge;
+    out[10] = e.mn; out[11] = (double)e.imn;
+    out[12] = e.mx; out[13] = (double)e.imx;
+    out[14] = e.ab; out[15] = (double)e.iab;
   }
 }
 
-static RedCoef red_coef(const pn_ctx *c) {
+static RedCoef red_coef(const pn_params &p, int a2_pow2) {
   RedCoef k;
   const int Nc = 1;
-  k.a2 = c->p.a * c->p.a;
-  k.cm = (c->p.mass_mode) * 0.5 * c->p.m2;
-  k.lmode = (double)c->p.lambda_mode;
-  k.d6 = 720.0 * Nc * Nc * c->p.kappa * c->p.kappa;
+  k.a2 = p.a * p.a;
+  k.cm = (p.mass_mode) * 0.5 * p.m2;
+  k.lmode = (double)p.lambda_mode;
+  k.d6 = 720.0 * Nc * Nc * p.kappa * p.kappa;
   k.inv_a2 = 1.0 / k.a2;
-  k.pow2 = c->k.a2_pow2;
+  k.pow2 = a2_pow2;
   return k;
 }
 
-// runs K3/K4 and leaves 9 sums + 6 extrema values in c->host_pin
+static RedCoef red_coef(const pn_ctx *c) { return red_coef(c->p, c->k.a2_pow2); }
+
+static VolCoef vol_coef(const pn_params &p, int a2_pow2) {
+  VolCoef k;
+  k.r = red_coef(p, a2_pow2);
+  // the fused gradient scaling needs 1/a2 in [2^-200, 2^200]; otherwise divide
+  if (!(k.r.inv_a2 >= 0x1p-200 && k.r.inv_a2 <= 0x1p+200)) k.r.pow2 = 0;
+  int f24 = 0, fd6 = 0;
+  k.r24 = pn_make_recip(24.0, &f24);
+  k.rd6 = pn_make_recip(k.r.d6, &fd6);
+  k.fdiv = f24 && fd6 && pn_bmr_exact(k.r24);
+  k.d6_bmr = k.fdiv && pn_bmr_exact(k.rd6);
+  return k;
+}
+
+// blocks of k_volume<T, W, POW2> resident at once on device dev (capped at
+// the partials' room), found once per device
+template <typename T, int W, bool POW2, bool FASTOK>
+static int vol_grid(int dev) {
+  static std::mutex mu;
+  static int cached[kScratchMaxDevices];
+  std::lock_guard<std::mutex> lk(mu);
+  if (dev < 0 || dev >= kScratchMaxDevices) return 256;
+  if (!cached[dev]) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_volume<T, W, POW2, FASTOK>,
+                                                     kRedThreads, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    cached[dev] = std::max(1, std::min(kVolMaxBlocks, per_cu * cus));
+  }
+  return cached[dev];
+}
+
+template <typename T, int W, bool POW2, bool FASTOK>
+static hipError_t launch_volume_t(const pn_params &p, const VolCoef &k, const void *phi,
+                                  const void *pi, hipStream_t st, const VolScratch &sc, double *out) {
+  const int grid = vol_grid<T, W, POW2, FASTOK>(p.device);
+  const VolShape s = vol_shape(p.N, W, grid);
+  const int nb = std::min(s.tiles, grid);
+  hipLaunchKernelGGL((k_volume<T, W, POW2, FASTOK>), dim3(nb), dim3(kRedThreads), 0, st,
+                     (const T *)phi, (const T *)pi, p.N, s, k, sc.part, sc.epart, sc.tflag, sc.bflag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const double vol = (double)((long long)p.N * p.N * p.N);  // N^3 overflows int above 1290
+  hipLaunchKernelGGL((k_volume_final<T, W, POW2>), dim3(1), dim3(kVolFinalThreads), 0, st,
+                     (const T *)phi, (const T *)pi, p.N, s, k, sc.part, sc.epart, nb, sc.tflag,
+                     sc.bflag, vol, out);
+  return hipGetLastError();
+}
+
+template <typename T, int W>
+static hipError_t launch_volume_w(const pn_params &p, const VolCoef &k, const void *phi,
+                                  const void *pi, hipStream_t st, const VolScratch &sc, double *out) {
+  if (!k.fdiv)
+    return k.r.pow2 ? launch_volume_t<T, W, true, false>(p, k, phi, pi, st, sc, out)
+                    : launch_volume_t<T, W, false, false>(p, k, phi, pi, st, sc, out);
+  return k.r.pow2 ? launch_volume_t<T, W, true, true>(p, k, phi, pi, st, sc, out)
+                  : launch_volume_t<T, W, false, true>(p, k, phi, pi, st, sc, out);
+}
+
+// Enqueues K3/K4 on stream st over dense fields phi, pi of p.N^3 values
+// (p.precision's type); the kVolOut results go to out.  Vector loads of W
+// sites need N divisible by W and W-aligned pointers.
+static hipError_t launch_volume(const pn_params &p, int a2_pow2, const void *phi, const void *pi,
+                                hipStream_t st, const VolScratch &sc, double *out) {
+  const VolCoef k = vol_coef(p, a2_pow2);
+  const size_t es = p.precision == PN_FP64 ? 8 : 4;
+  const uintptr_t al = (uintptr_t)phi | (uintptr_t)pi;
+  int W = p.precision == PN_FP64 ? 2 : 4;
+  while (W > 1 && (p.N % W != 0 || al % (W * es) != 0)) W >>= 1;
+  if (p.precision == PN_FP64)
+    return W == 2 ? launch_volume_w<double, 2>(p, k, phi, pi, st, sc, out)
+                  : launch_volume_w<double, 1>(p, k, phi, pi, st, sc, out);
+  if (W == 4) return launch_volume_w<float, 4>(p, k, phi, pi, st, sc, out);
+  if (W == 2) return launch_volume_w<float, 2>(p, k, phi, pi, st, sc, out);
+  return launch_volume_w<float, 1>(p, k, phi, pi, st, sc, out);
+}
+
+static VolScratch vol_scratch(const pn_ctx *c) {
+  VolScratch sc;
+  sc.part = c->red_part;
+  sc.epart = c->ext_part;
+  sc.tflag = c->tile_flag;
+  sc.bflag = c->tile_flag + kVolMaxTiles;
+  return sc;
+}
+
+// runs K3/K4 and leaves the kVolOut results in c->host_pin
 static int run_volume(pn_ctx *c) {
   PN_HIP(c, hipSetDevice(c->p.device));
-  RedCoef k = red_coef(c);
-  const int nb = c->red_blocks;
-  if (c->p.precision == PN_FP64)
-    hipLaunchKernelGGL(k_volume<double>, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                       (const double *)c->phi[c->cur], (const double *)c->pi, c->p.N, k,
-                       c->red_part, c->ext_part);
-  else
-    hipLaunchKernelGGL(k_volume<float>, dim3(nb), dim3(kRedThreads), 0, c->stream,
-                       (const float *)c->phi[c->cur], (const float *)c->pi, c->p.N, k,
-                       c->red_part, c->ext_part);
-  PN_CHECK_LAUNCH(c);
-  hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c->stream, c->red_part, c->ext_part,
-                     nb, c->red_out);
-  PN_CHECK_LAUNCH(c);
-  PN_HIP(c, hipMemcpyAsync(c->host_pin, c->red_out, (kNSums + 6) * sizeof(double),
+  PN_HIP(c, launch_volume(c->p, c->k.a2_pow2, c->phi[c->cur], c->pi, c->stream, vol_scratch(c),
+                          c->red_out));
+  PN_HIP(c, hipMemcpyAsync(c->host_pin, c->red_out, kVolOut * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
   PN_HIP(c, hipStreamSynchronize(c->stream));
   return PN_OK;
+}
+
+static void vol_unpack(const double *s, pn_averages *avg, pn_extrema *ext) {
+  if (avg) {
+    avg->phi = s[0]; avg->pi = s[1]; avg->phi2 = s[2]; avg->pi2 = s[3]; avg->phipi = s[4];
+    avg->phi4 = s[5]; avg->phi6 = s[6]; avg->energy = s[7]; avg->interaction = s[8];
+    avg->charge = s[9];
+  }
+  if (ext) {
+    ext->min = s[10]; ext->minpos = (long long)s[11];
+    ext->max = s[12]; ext->maxpos = (long long)s[13];
+    ext->maxabs = s[14]; ext->maxabspos = (long long)s[15];
+  }
 }
 
 extern "C" int pn_volume_averages(pn_ctx *c, pn_averages *out) {
   if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_volume_averages: NULL argument");
   int rc = run_volume(c);
   if (rc != PN_OK) return rc;
-  const double *s = c->host_pin;
-  const long long vol = (long long)c->p.N * c->p.N * c->p.N;  // N^3 overflows int above 1290
-  // final arithmetic as fieldanalysis.cpp:619-638 (one rank: /nodes == /1)
-  double phi1 = s[0], pi1 = s[1], phi2 = s[2], pi2 = s[3], phipi = s[4], phi4 = s[5], phi6 = s[6];
-  double G = s[7], I = s[8];
-  phi1 /= (vol); pi1 /= (vol);
-  double Charge = (0.5 * pi2 - G / 6. - I);
-  double Energy = (0.5 * pi2 + 0.5 * G + I);
-  Energy /= (vol); I /= (vol); Charge /= (vol);
-  phi2 /= (vol); pi2 /= (vol); phi4 /= (vol); phi6 /= (vol); phipi /= (vol);
-  out->phi = phi1; out->pi = pi1; out->phi2 = phi2; out->pi2 = pi2; out->phipi = phipi;
-  out->phi4 = phi4; out->phi6 = phi6; out->energy = Energy; out->interaction = I;
-  out->charge = Charge;
+  vol_unpack(c->host_pin, out, nullptr);
   return PN_OK;
 }
 
 extern "C" int pn_volume_analysis(pn_ctx *c, pn_averages *avg, pn_extrema *ext) {
   if (!c) return pn_fail(c, PN_ERR_ARG, "pn_volume_analysis: NULL context");
-  int rc = PN_OK;
-  if (avg && (rc = pn_volume_averages(c, avg)) != PN_OK) return rc;  // one k_volume run
-  if (ext) {
-    if (!avg && (rc = run_volume(c)) != PN_OK) return rc;
-    const double *s = c->host_pin + kNSums;
-    ext->min = s[0]; ext->minpos = (long long)s[1];
-    ext->max = s[2]; ext->maxpos = (long long)s[3];
-    ext->maxabs = s[4]; ext->maxabspos = (long long)s[5];
-  }
+  if (!avg && !ext) return PN_OK;
+  int rc = run_volume(c);  // one k_volume run for both
+  if (rc != PN_OK) return rc;
+  vol_unpack(c->host_pin, avg, ext);
   return PN_OK;
 }
 
@@ -1208,10 +1535,7 @@ extern "C" int pn_extrema_get(pn_ctx *c, pn_extrema *out) {
   if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_extrema_get: NULL argument");
   int rc = run_volume(c);
   if (rc != PN_OK) return rc;
-  const double *s = c->host_pin + kNSums;
-  out->min = s[0]; out->minpos = (long long)s[1];
-  out->max = s[2]; out->maxpos = (long long)s[3];
-  out->maxabs = s[4]; out->maxabspos = (long long)s[5];
+  vol_unpack(c->host_pin, nullptr, out);
   return PN_OK;
 }
 

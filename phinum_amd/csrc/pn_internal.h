@@ -54,6 +54,7 @@ struct pn_ctx {
   double *red_part;    // per-block partial sums
   double *red_out;     // final sums (device)
   long long *ext_part; // per-block extrema
+  int *tile_flag;      // per-tile flags of the volume sums' main pass
   double *host_pin;    // pinned host staging for small results
   int red_blocks;
   pn_fft_state *fft;

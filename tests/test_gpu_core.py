@@ -408,6 +408,54 @@ def test_volume_analysis_is_the_two_calls(N, prec):
         assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
 
 
+@pytest.mark.parametrize("N,prec,kappa", [(33, pn.PN_FP64, None), (64, pn.PN_FP64, None),
+                                           (64, pn.PN_FP32, None), (40, pn.PN_FP64, 1e300)])
+def test_volume_sums_outside_fast_range(N, prec, kappa):
+    # waves holding zeros, -0, subnormals, tiny (< 2^-100) or huge (> 2^100)
+    # sites take the IEEE divisions and square root in a second pass; kappa =
+    # 1e300 puts the phi^6 divisor outside the fast form's range, so every
+    # site does; values against the oracle, extrema bitwise
+    m = O.model(N)
+    if kappa is not None:
+        m = dict(m, kappa=kappa)
+    phi, pi = _fields(N, 70 + N)
+    rng = np.random.default_rng(N)
+    sites = rng.choice(N ** 3, 40, replace=False)
+    special = [0.0, -0.0, 5e-320, -1e-310, 1e-120, 2.0 ** 120, -3.0 ** 70, 1e-35]
+    for k, i in enumerate(sites):
+        phi[i] = special[k % len(special)]
+    if prec == pn.PN_FP32:
+        phi = phi.astype(np.float32).astype(np.float64)
+        pi = pi.astype(np.float32).astype(np.float64)
+    ref = O.volume_averages(m, phi, pi)
+    vals, idx = O.extrema(N, phi)
+    with _lat(m, prec) as L:
+        L.set_fields(phi, pi)
+        avg, ext = L.volume_analysis()
+        again, _ = L.volume_analysis()
+    assert np.array_equal(_bits(avg.as_array()), _bits(again.as_array()))
+    np.testing.assert_allclose(avg.as_array(), ref, rtol=1e-13, atol=1e-15)
+    assert _bits(np.array([ext.min, ext.max, ext.maxabs])).tolist() == _bits(np.array(vals)).tolist()
+    assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
+
+
+def test_extrema_of_signed_zeros():
+    # all zeros with -0 at the first site: the reference's strict comparisons
+    # keep the first site's -0 as both extremes (fieldanalysis.cpp:331-355)
+    N = 16
+    m = O.model(N)
+    phi = np.zeros(N ** 3)
+    phi[0] = -0.0
+    phi[N ** 3 // 3] = -0.0
+    pi = np.zeros(N ** 3)
+    vals, idx = O.extrema(N, phi)
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        ext = L.extrema()
+    assert _bits(np.array([ext.min, ext.max, ext.maxabs])).tolist() == _bits(np.array(vals)).tolist()
+    assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
+
+
 def test_extrema_ties_pick_first_site():
     N = 8
     m = O.model(N)
