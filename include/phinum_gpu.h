@@ -218,6 +218,22 @@ int pn_extrema_get(pn_ctx *c, pn_extrema *out);
  * (fieldanalysis.cpp:748-799) calls them back to back; either output may be
  * NULL.  The values are those of the two separate calls. */
 int pn_volume_analysis(pn_ctx *c, pn_averages *avg, pn_extrema *ext);
+/* pn_volume_analysis without allocating (no reference counterpart): for
+ * callers that cannot call hipMalloc, such as code under HIP graph capture or
+ * a serving loop.  phi and pi are the caller's dense device fields of N^3
+ * values (double for PN_FP64, float for PN_FP32) on device p->device; the
+ * launches are enqueued on `stream` (a hipStream_t; NULL: the null stream)
+ * and nothing waits for them.  out is a device array of PN_VOLUME_OUT doubles
+ * that receives the pn_averages fields in their order, then min, minpos, max,
+ * maxpos, maxabs, maxabspos (positions as doubles).  The values are those of
+ * pn_volume_analysis on the same fields.  The block partials and tile flags go
+ * to a static device array of 448 KiB per device (at most 2048 blocks),
+ * so N must lie in [4, PN_NOALLOC_MAX_N]; calls that overlap on one device
+ * share that array and race.  Errors are reported by pn_last_error(NULL). */
+#define PN_VOLUME_OUT 16
+#define PN_NOALLOC_MAX_N 2048
+int pn_volume_analysis_noalloc(const pn_params *p, const void *phi, const void *pi, void *stream,
+                               double *out);
 /* nTBins of momentumspace.cpp:40-47 for this N and a. */
 int pn_spectrum_nbins(int N, double a);
 /* Calculate_CorrelationFunctions (momentumspace.cpp:140-278) with

@@ -28,6 +28,8 @@ PN_OPT_STEP2_ZBLOCK = 3
 PN_OPT_THREE_STEP = 4
 PN_OPT_STEP3_ZBLOCK = 5
 PN_OPT_KERNEL_TIMED = 6
+PN_VOLUME_OUT = 16
+PN_NOALLOC_MAX_N = 2048
 
 
 class PnParams(C.Structure):
@@ -136,6 +138,8 @@ def lib():
         "pn_volume_averages": (C.c_int, [C.c_void_p, P(PnAverages)]),
         "pn_extrema_get": (C.c_int, [C.c_void_p, P(PnExtrema)]),
         "pn_volume_analysis": (C.c_int, [C.c_void_p, P(PnAverages), P(PnExtrema)]),
+        "pn_volume_analysis_noalloc": (C.c_int, [P(PnParams), C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]),
         "pn_spectrum_nbins": (C.c_int, [C.c_int, C.c_double]),
         "pn_spectrum_get": (C.c_int, [C.c_void_p, P(PnSpectrum)]),
         "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),

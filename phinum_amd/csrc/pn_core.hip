@@ -1540,6 +1540,77 @@ extern "C" int pn_extrema_get(pn_ctx *c, pn_extrema *out) {
 }
 
 // ---------------------------------------------------------------------------
+// volume analysis that allocates nothing (no reference counterpart)
+// ---------------------------------------------------------------------------
+// For callers that cannot allocate (code under HIP graph capture, serving
+// loops): the block partials of K3/K4 live in a file-scope device array
+// instead of context memory.  Each device holds its own copy of the array at
+// its own address, looked up once per device with that device current; the
+// launches take that address, never the array's host-side name.  Every call on
+// a device shares its copy, so calls that overlap on one device race.
+// Nothing in it needs clearing: the main pass writes every partial and every
+// tile flag the final pass reads, earlier in the same stream.  Layout: the
+// block partials (2 * kNSums doubles and 6 extrema words per block), then the
+// tile and block flags (two per double): 456 KiB
+constexpr size_t kScratchDoubles =
+    (size_t)kVolMaxBlocks * (2 * kNSums + 6) + (kVolMaxTiles + kVolMaxBlocks) / 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kScratchMaxDevices];  // device address of `scratch` per device
+
+// the device address of this device's copy (the device must be current)
+static int scratch_on(int dev, double **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return pn_fail(nullptr, PN_ERR_HIP, "pn_volume_analysis_noalloc: hipGetSymbolAddress on device %d: %s",
+                     dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return PN_OK;
+}
+
+extern "C" int pn_volume_analysis_noalloc(const pn_params *p, const void *phi, const void *pi,
+                                          void *stream, double *out) {
+  if (!p || !phi || !pi || !out)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_volume_analysis_noalloc: NULL argument");
+  if (p->N < 4 || p->N > PN_NOALLOC_MAX_N)
+    return pn_fail(nullptr, PN_ERR_ARG,
+                   "pn_volume_analysis_noalloc: N=%d outside [4, %d] (the static scratch holds the "
+                   "partials of at most %d blocks)", p->N, PN_NOALLOC_MAX_N, kVolMaxBlocks);
+  if (p->precision != PN_FP64 && p->precision != PN_FP32)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_volume_analysis_noalloc: unknown precision %d", p->precision);
+  if (!(p->a > 0) || !std::isfinite(p->a) || !(p->kappa != 0) || !std::isfinite(p->kappa))
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_volume_analysis_noalloc: a must be > 0 and Kappa finite and nonzero");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || p->device < 0 || p->device >= ndev ||
+      p->device >= kScratchMaxDevices)
+    return pn_fail(nullptr, PN_ERR_ARG, "pn_volume_analysis_noalloc: device %d of %d", p->device, ndev);
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(p->device) != hipSuccess)
+    return pn_fail(nullptr, PN_ERR_HIP, "pn_volume_analysis_noalloc: cannot select device %d", p->device);
+  double *sc = nullptr;
+  int rc = scratch_on(p->device, &sc);
+  if (rc == PN_OK) {
+    const double a2 = p->a * p->a;
+    VolScratch vs;
+    vs.part = sc;
+    vs.epart = (long long *)(sc + (size_t)kVolMaxBlocks * 2 * kNSums);
+    vs.tflag = (int *)(sc + (size_t)kVolMaxBlocks * (2 * kNSums + 6));
+    vs.bflag = vs.tflag + kVolMaxTiles;
+    const hipError_t e = launch_volume(*p, is_pow2_double(a2), phi, pi, (hipStream_t)stream, vs, out);
+    if (e != hipSuccess)
+      rc = pn_fail(nullptr, PN_ERR_HIP, "pn_volume_analysis_noalloc: launch failed: %s", hipGetErrorString(e));
+  }
+  (void)hipSetDevice(prev);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // per-site energy density (the summand of Energy, fieldanalysis.cpp:625)
 // ---------------------------------------------------------------------------
 template <typename T>
