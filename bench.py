@@ -1,21 +1,32 @@
 """Benchmark of the lattice hot path (one leapfrog step = kick + drift + wrap).
 
-Workload: the 256^3 FP64 drop-in configuration (dx = 0.5, dt = 0.05,
-V = phi^2/2 - phi^4/4 + 0.2 phi^6/6 in the reference's units, 10^4 steps with
-energy outputs every 200 steps).  `value` is FP64 lattice site-updates per
-second with the fields resident in HBM, timed with HIP events on the stream
-the step kernels run on.  The steps run as the multi-step (temporally
-blocked) kernel the library picks by timing during the warmup (two or three
-steps per pass over HBM, bit-identical to single steps); `roofline` reports
-that kernel against its own algorithmic traffic (Phi and Pi read and written
-once per launch) and, beside it, the site-update rate against the one-step
-bound (32 B per site-update).  The line also carries the single-step kernel, the
-FP32 form (FP32 fields, FP64 compensated energy sums) with its energy
-difference to the FP64 run, the 512^3 case, the batched ensemble, the
-adjoint, and the CPU baseline.
+Workload (the line's `value`): the precision form's lattice, 1024^3 FP64
+(dx = 0.5, dt = 0.05, V = phi^2/2 - 1.2 phi^4/4 + 0.25 phi^6/6 in the
+reference's units, phi0 = 1.2 plus seeded vacuum fluctuations), K timed steps
+with the fields resident in HBM, timed with HIP events on the stream the step
+kernels run on.  Before the timed steps the library's timed choice of the
+multi-step kernel (two or three leapfrog steps per pass over HBM, the same
+bits as single steps) runs to completion on untimed steps, whatever --warmup
+is; the line reports how many steps it took.  `roofline` reports the
+dominant kernel against its own algorithmic traffic (Phi and Pi read and
+written once per launch) and, beside it, the site-update rate against the
+one-step bound (32 B per site-update).
+
+The same FP64 lattice, an FP32 lattice from the same initial condition (FP32
+fields, FP64 compensated sums) and an FP64 lattice started one ulp away run
+on to --precision-steps steps: the FP32 form's rate over the same K steps,
+its ratio to FP64, and per output its energy difference to the FP64 run with
+the reference's staggered energy and with the time-synchronised one, field
+and spectrum errors, beside the one-ulp control.
+
+Extras: 256^3 and 512^3 FP64 and FP32 rates, the one-step kernel, the
+reference's loop at 256^3 with its outputs (averages, spectrum, island
+tracking every 200 steps), the batch of 64 realizations, the adjoint, and the
+CPU baseline (the reference's own evolve() on the host cores).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-extra]
                 [--kernel S:Z  (fix S steps per pass with z block Z instead of the timed choice)]
+                [--analysis N  (time the analysis calls at N^3 only)]
 """
 import argparse
 import json
@@ -46,58 +57,80 @@ def model(lam=1.0, g=0.2, phi0=1.0):
     units (Lambda = 6 lam, Kappa^2 = Lambda^2 / (120 g), phi_ref = sqrt(Lambda) phi)."""
     L = 6.0 * lam
     return dict(a=0.5, mass_mode=1, m2=1.0, lambda_mode=-1, kappa=float(np.sqrt(L * L / (120.0 * g))),
-                phi0=float(np.sqrt(L)) * phi0, Lambda=L)
+                phi0=float(np.sqrt(L)) * phi0, Lambda=L, lam=lam, g=g)
 
 
-def launches(steps, chunk, per_pass):
-    """Launch counts for `steps` run in chunks of `chunk` with the kernel
-    advancing `per_pass` (1, 2 or 3) steps per launch: (main launches, steps
-    left to shorter launches).  pn_evolve runs the rest of a chunk by pairs,
-    then singly."""
-    full, rest = divmod(steps, chunk)
-    sizes = [chunk] * full + ([rest] if rest else [])
-    return sum(n // per_pass for n in sizes), sum(n % per_pass for n in sizes)
+def lattice(pn, N, m, precision):
+    return pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"], precision=precision)
 
 
-def run_leg(pn, N, precision, steps, warmup, out_every, seed, m, two_step=True, kernel=None):
-    """Seeded initialize==3 IC on the device, then `steps` timed steps in chunks
-    of `out_every` with an FP64 energy between chunks (outside the timing).
-    The multi-step kernel is the library's own choice (timed during a warmup of
-    at least 118 steps) unless `kernel` = (steps per pass, z block) fixes it.
-    Returns (total event ms over `steps`, energies at each output, the kernel
-    used as {"steps_per_pass", "zblock"})."""
-    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], warmup + steps)
-    energies = []
-    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"],
-                    precision=precision) as L:
+def launch_count(chunks, per_pass):
+    """Launches for steps run as evolve calls of the given sizes with a kernel
+    advancing `per_pass` steps per launch: (multi-step launches, steps left to
+    shorter launches).  pn_evolve runs the rest of a call by pairs, then singly."""
+    return sum(n // per_pass for n in chunks), sum(n % per_pass for n in chunks)
+
+
+def kernel_used(pn, L):
+    if not L.two_step():
+        return {"steps_per_pass": 1, "zblock": None}
+    if L.three_step():
+        zb = C.c_int()
+        L._chk(L.L.pn_get_option(L.h, pn.PN_OPT_STEP3_ZBLOCK, C.byref(zb)))
+        return {"steps_per_pass": 3, "zblock": zb.value}
+    return {"steps_per_pass": 2, "zblock": L.step2_zblock()}
+
+
+def fix_kernel(L, kernel):
+    if kernel[0] == 3:
+        L.three_step(True, zblock=kernel[1])
+    else:
+        L.three_step(False)
+        L.step2_zblock(kernel[1])
+
+
+def tune(lats, dtk, dtd, s, warmup, stop=None, chunk=64):
+    """Advances the lattices together, untimed, until each has made its timed
+    choice of the multi-step kernel (PN_OPT_KERNEL_TIMED) and at least
+    `warmup` steps have run; evolve calls end at multiples of `stop` (the
+    output grid) when given.  Returns the step reached."""
+    s0 = s
+    while any(L.two_step() and not L.kernel_timed() for L in lats) or s - s0 < warmup:
+        n = chunk
+        if stop:
+            n = min(n, stop - s % stop)
+        for L in lats:
+            L.evolve(dtk[s:s + n], dtd[s:s + n])
+        s += n
+        if s - s0 > 4000:
+            raise RuntimeError("the kernel choice did not finish in 4000 steps")
+    return s
+
+
+def run_leg(pn, N, precision, steps, warmup, seed, m, two_step=True, kernel=None, chunk=None):
+    """Seeded initialize==3 IC on the device, the timed kernel choice on
+    untimed steps, then `steps` timed steps (in evolve calls of `chunk` steps,
+    or one call).  Returns (event ms over the timed steps, the evolve call
+    sizes, kernel used, untimed steps before the timing)."""
+    tune_chunk = 64
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], 4000 + warmup + steps)
+    with lattice(pn, N, m, precision) as L:
         if not two_step:
             L.two_step(False)
         elif kernel:
-            if kernel[0] == 3:
-                L.three_step(True, zblock=kernel[1])
-            else:
-                L.three_step(False)
-                L.step2_zblock(kernel[1])
+            fix_kernel(L, kernel)
         L.initial_conditions(pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
-        if warmup:
-            L.evolve(dtk[:warmup], dtd[:warmup])
-        energies.append(L.volume_averages().energy)
-        total = 0.0
-        s = warmup
-        while s < warmup + steps:
-            n = min(out_every, warmup + steps - s)
+        s = tune([L], dtk, dtd, 0, warmup, chunk=tune_chunk)
+        chunks, total = [], 0.0
+        left = steps
+        while left:
+            n = min(left, chunk or left)
             total += L.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+            chunks.append(n)
             s += n
-            energies.append(L.volume_averages().energy)
-        if not L.two_step():
-            used = {"steps_per_pass": 1, "zblock": None}
-        elif L.three_step():
-            zb = C.c_int()
-            L._chk(L.L.pn_get_option(L.h, pn.PN_OPT_STEP3_ZBLOCK, C.byref(zb)))
-            used = {"steps_per_pass": 3, "zblock": zb.value}
-        else:
-            used = {"steps_per_pass": 2, "zblock": L.step2_zblock()}
-    return total, np.array(energies), used
+            left -= n
+        used = kernel_used(pn, L)
+    return total, chunks, used, s - steps
 
 
 def profiled_value(name, key):
@@ -111,11 +144,6 @@ def profiled_value(name, key):
         return None
     m = re.search(r"%s = ([0-9.]+)" % key, open(path).read())
     return float(m.group(1)) if m else None
-
-
-def profiled_traffic(name):
-    t = profiled_value(name, "traffic_bytes_per_launch")
-    return int(t) if t else None
 
 
 def kernel_names(t, per_pass, N):
@@ -132,29 +160,29 @@ def kernel_names(t, per_pass, N):
     return "k_step_vec<%s, %s, POW2, W, 4, ZB, 0> (one fused step)" % (t, c), "k_step_vec_%s_%dcube" % (p, N)
 
 
-def roofline(kernel, N, ms_total, steps, chunk, per_pass, esize, profile=None):
+def roofline(kernel, N, ms_total, chunks, per_pass, esize, profile=None):
     """Roofline of the dominant kernel: algorithmic bytes per launch (Phi and
     Pi read once and written once: 4 * esize bytes per site) over the average
     launch duration, plus the site-update rate against the one-step bound."""
-    nmain, nrest = launches(steps, chunk, per_pass)
+    nmain, nrest = launch_count(chunks, per_pass)
+    steps = sum(chunks)
     sites = float(N) ** 3
     algo = 4.0 * esize * sites
-    # the few steps of a chunk left to shorter launches count pro rata
+    # the few steps left to shorter launches count pro rata
     per_launch_ms = ms_total / (nmain + nrest / float(per_pass))
-    steps_per_launch = per_pass
     achieved = algo / (per_launch_ms * 1e-3) / 1e9
     upd = sites * steps / (ms_total * 1e-3)
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
-           "steps_per_launch": steps_per_launch, "algorithmic_bytes_per_launch": int(algo),
-           "avg_launch_ms": per_launch_ms,
+           "steps_per_launch": per_pass, "algorithmic_bytes_per_launch": int(algo),
+           "launches": nmain, "avg_launch_ms": per_launch_ms,
            "one_step_bound": {"bytes_per_site_update": 4 * esize,
                               "site_updates_per_s_at_peak": HBM_PEAK_GBS * 1e9 / (4 * esize),
                               "frac": upd * 4 * esize / 1e9 / HBM_PEAK_GBS}}
     if profile:
-        t = profiled_traffic(profile)
+        t = profiled_value(profile, "traffic_bytes_per_launch")
         if t:
-            out["traffic"] = t
+            out["traffic"] = int(t)
             out["traffic_over_algorithmic"] = t / algo
             out["traffic_source"] = "profiles/%s.md" % profile
         flop = profiled_value(profile, "fp64_flop_per_launch")
@@ -169,83 +197,187 @@ def roofline(kernel, N, ms_total, steps, chunk, per_pass, esize, profile=None):
     return out
 
 
-def precision_leg(pn, N, steps, out_every, seed=1234):
-    """Precision form: N^3, lambda = 1.2, g = 0.25, phi0 = 1.2 plus seeded
-    vacuum fluctuations, `steps` steps.  Phi and Pi stored and stepped in FP32
-    (energy and spectrum sums in FP64, compensated) against the same run in
-    FP64 from the same initial condition.  Per output: relative energy
-    difference, max |dphi| (absolute and over max |phi|), relative L2 field
-    errors, and the spectrum's relative error (max over the shells below the
-    initial cutoff QS, where the initial condition put power, of
-    |F32 - F64| / F64).  A third run, FP64 from the FP64 initial condition with
-    every phi moved up by one ulp, measures how fast this chaotic system
-    separates trajectories that start 1e-16 apart."""
+def precision_leg(pn, N, K, warmup, total_steps, out_every, seed=1234, control=True, kernel=None):
+    """The headline and the precision form.  N^3, lambda = 1.2, g = 0.25,
+    phi0 = 1.2 plus seeded vacuum fluctuations.  An FP64 lattice, an FP32
+    lattice from the same initial condition and (control) an FP64 lattice
+    with every phi one ulp higher step together: untimed until both timed
+    kernel choices are made, then K steps timed on the FP64 and the FP32
+    lattice (evolve calls ending on the output grid), then on to total_steps.
+    Every out_every steps: energies (the reference's, with pi half a step
+    behind phi, and the time-synchronised one, pi kicked by dtau/2), the FP32
+    field's max and L2 errors against FP64, the spectrum's relative error
+    below QS, and the same for the control."""
     m = model(lam=1.2, g=0.25, phi0=1.2)
-    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], steps)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, m["a"], total_steps + 4000 + K + warmup)
+    half = 0.5 * 0.1 * m["a"]
     ic = pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"])
     rows = []
     ms = {"fp64": 0.0, "fp32": 0.0}
+    chunks = []
+    with lattice(pn, N, m, pn.PN_FP64) as L64, lattice(pn, N, m, pn.PN_FP32) as L32:
+        Lc = lattice(pn, N, m, pn.PN_FP64) if control else None
+        try:
+            if kernel:
+                fix_kernel(L64, kernel)
+            t0 = time.time()
+            L64.initial_conditions(ic)
+            L32.initial_conditions(ic)
+            log("precision %d^3: initial conditions in %.1fs" % (N, time.time() - t0))
+            if Lc is not None:
+                phi, pi = L64.get_fields()
+                Lc.set_fields(np.nextafter(phi, np.inf), pi)
+                del phi, pi
+            lats = [L for L in (L64, L32, Lc) if L is not None]
 
-    def lattice(prec):
-        return pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"], precision=prec)
+            def record(step):
+                r = dict(step=step)
+                e = {k: L.volume_averages().energy for k, L in (("64", L64), ("32", L32))}
+                # at step 0 pi and phi share the initial time: no kick
+                h = half if step > 0 else 0.0
+                es = {k: L.volume_averages_synced(h).energy for k, L in (("64", L64), ("32", L32))}
+                d = L32.field_diff(L64)
+                s64, s32 = L64.spectrum(), L32.spectrum()
+                p = np.arange(s64["nbins"]) * s64["binsize"]
+                sel = (s64["counts"] > 0) & (p < ic.QS)
+                r.update(e64=e["64"], e32=e["32"], dE=abs(e["32"] - e["64"]) / abs(e["64"]),
+                         e64_sync=es["64"], dE_sync=abs(es["32"] - es["64"]) / abs(es["64"]),
+                         max_dphi=d["max_dphi"], max_dphi_rel=d["max_dphi"] / d["max_phi"],
+                         l2_dphi=d["l2_dphi"], l2_dpi=d["l2_dpi"],
+                         spec=float(np.max(np.abs(s32["F"][sel] - s64["F"][sel]) / s64["F"][sel])))
+                if Lc is not None:
+                    dc = Lc.field_diff(L64)
+                    r.update(ulp_dE=abs(Lc.volume_averages().energy - e["64"]) / abs(e["64"]),
+                             ulp_dE_sync=abs(Lc.volume_averages_synced(h).energy - es["64"]) / abs(es["64"]),
+                             ulp_l2_dphi=dc["l2_dphi"], ulp_max_dphi=dc["max_dphi"])
+                rows.append(r)
+                log("precision %d^3 step %d: fp32 dE/E %.3g (synced %.3g), max|dphi| %.3g, L2 %.3g, "
+                    "spectrum %.3g" % (N, step, r["dE"], r["dE_sync"], d["max_dphi"], d["l2_dphi"], r["spec"]))
 
-    with lattice(pn.PN_FP64) as L64, lattice(pn.PN_FP32) as L32, lattice(pn.PN_FP64) as Lc:
-        t0 = time.time()
-        L64.initial_conditions(ic)
-        L32.initial_conditions(ic)
-        log("precision %d^3: initial conditions in %.1fs" % (N, time.time() - t0))
-        phi, pi = L64.get_fields()
-        Lc.set_fields(np.nextafter(phi, np.inf), pi)
-        del phi, pi
+            record(0)
+            s = tune(lats, dtk, dtd, 0, warmup, stop=out_every)
+            if s % out_every == 0:
+                record(s)
+            tuned_at = s
+            log("precision %d^3: kernel choice done after %d untimed steps" % (N, s))
+            timed_end = s + K
+            while s < max(total_steps, timed_end):
+                n = min(out_every - s % out_every, max(total_steps, timed_end) - s)
+                if s < timed_end:
+                    n = min(n, timed_end - s)
+                    ms["fp64"] += L64.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+                    ms["fp32"] += L32.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+                    chunks.append(n)
+                    if Lc is not None:
+                        Lc.evolve(dtk[s:s + n], dtd[s:s + n])
+                else:
+                    for L in lats:
+                        L.evolve(dtk[s:s + n], dtd[s:s + n])
+                s += n
+                if s % out_every == 0:
+                    record(s)
+            k64, k32 = kernel_used(pn, L64), kernel_used(pn, L32)
+        finally:
+            if Lc is not None:
+                Lc.close()
+    sites = float(N) ** 3
+    v64 = sites * K / (ms["fp64"] * 1e-3)
+    v32 = sites * K / (ms["fp32"] * 1e-3)
 
-        def record(step):
-            e64, e32, ec = (L.volume_averages().energy for L in (L64, L32, Lc))
-            d = L32.field_diff(L64)
-            dc = Lc.field_diff(L64)
-            s64, s32 = L64.spectrum(), L32.spectrum()
-            p = np.arange(s64["nbins"]) * s64["binsize"]
-            sel = (s64["counts"] > 0) & (p < ic.QS)
-            spec = float(np.max(np.abs(s32["F"][sel] - s64["F"][sel]) / s64["F"][sel]))
-            rows.append(dict(step=step, e64=e64, e32=e32, dE=abs(e32 - e64) / abs(e64),
-                             max_dphi=d["max_dphi"], max_dphi_rel=d["max_dphi"] / d["max_phi"],
-                             l2_dphi=d["l2_dphi"], l2_dpi=d["l2_dpi"], spec=spec,
-                             ulp_dE=abs(ec - e64) / abs(e64), ulp_l2_dphi=dc["l2_dphi"],
-                             ulp_max_dphi=dc["max_dphi"]))
-            log("precision %d^3 step %d: fp32 dE/E %.3g, max|dphi| %.3g, L2 %.3g, spectrum %.3g; "
-                "fp64+1ulp dE/E %.3g, L2 %.3g" % (N, step, rows[-1]["dE"], d["max_dphi"], d["l2_dphi"], spec,
-                                                   rows[-1]["ulp_dE"], dc["l2_dphi"]))
+    def within(key):
+        bad = [r["step"] for r in rows if r[key] > 1e-6]
+        return int(s if not bad else max([r["step"] for r in rows if r["step"] < bad[0]], default=0))
 
-        record(0)
-        s = 0
-        while s < steps:
-            n = min(out_every, steps - s)
-            ms["fp64"] += L64.evolve_timed(dtk[s:s + n], dtd[s:s + n])
-            ms["fp32"] += L32.evolve_timed(dtk[s:s + n], dtd[s:s + n])
-            Lc.evolve(dtk[s:s + n], dtd[s:s + n])
-            s += n
-            record(s)
-        on64, on32 = ("three-step" if L.three_step() else "two-step" if L.two_step() else "one-step"
-                      for L in (L64, L32))
-    dE = np.array([r["dE"] for r in rows])
-    over = np.nonzero(dE > 1e-6)[0]
+    keys = ["step", "dE", "dE_sync", "max_dphi", "max_dphi_rel", "l2_dphi", "l2_dpi", "spec"]
+    if control:
+        keys += ["ulp_dE", "ulp_dE_sync", "ulp_l2_dphi", "ulp_max_dphi"]
     e64 = np.array([r["e64"] for r in rows])
-    e32 = np.array([r["e32"] for r in rows])
-    v64 = float(N) ** 3 * steps / (ms["fp64"] * 1e-3)
-    v32 = float(N) ** 3 * steps / (ms["fp32"] * 1e-3)
-    return {"N": N, "steps": steps, "out_every": out_every, "lambda": 1.2, "g": 0.25, "phi0": 1.2,
-            "seed": seed, "fp64_kernel": on64, "fp32_kernel": on32,
-            "fp64_ms_per_step": ms["fp64"] / steps, "fp32_ms_per_step": ms["fp32"] / steps,
-            "fp64_site_updates_per_s": v64, "fp32_site_updates_per_s": v32, "ratio_fp32_vs_fp64": v32 / v64,
-            "steps_with_dE_over_E_le_1e-6": int(steps if over.size == 0 else rows[over[0] - 1]["step"]),
-            "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
-            "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
-            "per_output": {k: [float("%.3g" % r[k]) for r in rows]
-                           for k in ("step", "dE", "max_dphi", "max_dphi_rel", "l2_dphi", "l2_dpi", "spec",
-                                     "ulp_dE", "ulp_l2_dphi", "ulp_max_dphi")},
-            "per_output_keys": "dE = |E32 - E64| / E64; max_dphi = max |phi32 - phi64|; max_dphi_rel = "
-                               "max_dphi / max |phi64|; l2_dphi, l2_dpi = relative L2 field errors; spec = "
-                               "max over shells below QS of |F32 - F64| / F64; ulp_* = the same for an FP64 "
-                               "run started with every phi one ulp higher, against the FP64 run"}
+    return {
+        "fp64": {"ms": ms["fp64"], "chunks": chunks, "kernel": k64, "value": v64},
+        "fp32": {"ms": ms["fp32"], "chunks": chunks, "kernel": k32, "value": v32},
+        "untimed_steps_before_timing": tuned_at,
+        "table": {"N": N, "steps": s, "out_every": out_every, "lambda": 1.2, "g": 0.25, "phi0": 1.2,
+                  "seed": seed, "fp64_kernel": k64, "fp32_kernel": k32,
+                  "fp64_ms_per_step": ms["fp64"] / K, "fp32_ms_per_step": ms["fp32"] / K,
+                  "fp64_site_updates_per_s": v64, "fp32_site_updates_per_s": v32,
+                  "ratio_fp32_vs_fp64": v32 / v64,
+                  "steps_with_dE_over_E_le_1e-6": within("dE"),
+                  "steps_with_synced_dE_over_E_le_1e-6": within("dE_sync"),
+                  "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
+                  "per_output": {k: [float("%.3g" % r[k]) for r in rows] for k in keys},
+                  "per_output_keys": "dE = |E32 - E64| / E64 with the reference's energy (pi half a step "
+                                     "behind phi); dE_sync = the same with pi kicked by dtau/2 to phi's "
+                                     "time; max_dphi = max |phi32 - phi64|; max_dphi_rel = max_dphi / "
+                                     "max |phi64|; l2_dphi, l2_dpi = relative L2 field errors; spec = max "
+                                     "over shells below QS of |F32 - F64| / F64; ulp_* = the same for an "
+                                     "FP64 run started with every phi one ulp higher, against the FP64 run"}}
+
+
+def cfg1_leg(pn, N=256, steps=10000, out_every=200, seed=1234):
+    """The reference's time loop at N^3 FP64 (main.cpp:52-96 with the config's
+    cadence): every out_every steps the averages and extrema
+    (Print_AveragesExtrema), the shell spectrum (Print_fp) and an island
+    analysis with tracking (Full_Analysis at 2.57583 sqrt(<phi^2>),
+    main.cpp:70), no files.  Steps timed with events, each output on the
+    host clock (the calls return their results)."""
+    m = model()
+    dtk, dtd, taus = pn.time_factors(0.0, 0.1, m["a"], 4000 + steps)
+    t_out = {"volume": 0.0, "spectrum": 0.0, "islands": 0.0}
+    with lattice(pn, N, m, pn.PN_FP64) as L, pn.Tracker() as T:
+        L.initial_conditions(pn.ic_params(seed, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
+        s0 = tune([L], dtk, dtd, 0, 0)
+        s = s0
+        ms_steps = 0.0
+        n_out = 0
+        t_wall = time.perf_counter()
+        while s < s0 + steps:
+            n = min(out_every, s0 + steps - s)
+            ms_steps += L.evolve_timed(dtk[s:s + n], dtd[s:s + n])
+            s += n
+            t0 = time.perf_counter()
+            avg, ext = L.volume_analysis()
+            t1 = time.perf_counter()
+            L.spectrum()
+            t2 = time.perf_counter()
+            T.update(L, 0, 2.57583 * np.sqrt(avg.phi2), taus[s - 1])
+            t3 = time.perf_counter()
+            t_out["volume"] += t1 - t0
+            t_out["spectrum"] += t2 - t1
+            t_out["islands"] += t3 - t2
+            n_out += 1
+        wall = time.perf_counter() - t_wall
+        tracked = len(T.islands())
+        used = kernel_used(pn, L)
+    return {"N": N, "steps": steps, "out_every": out_every, "outputs": n_out,
+            "ms_per_step_steps_only": ms_steps / steps,
+            "ms_per_step_with_outputs": wall * 1e3 / steps,
+            "ms_per_output": {k: v * 1e3 / max(n_out, 1) for k, v in t_out.items()},
+            "site_updates_per_s_with_outputs": float(N) ** 3 * steps / wall,
+            "tracked_islands_at_end": tracked, "kernel": used}
+
+
+def analysis_only(pn, N, reps=10):
+    """Times each analysis call at N^3 FP64 (cfg1 model, 200 steps after a
+    seeded initial condition): for profiling the analysis kernels."""
+    m = model()
+    dtk, dtd, taus = pn.time_factors(0.0, 0.1, m["a"], 200)
+    out = {"N": N}
+    with lattice(pn, N, m, pn.PN_FP64) as L:
+        L.initial_conditions(pn.ic_params(1234, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
+        L.evolve(dtk, dtd)
+        thr = 2.57583 * np.sqrt(L.volume_averages().phi2)
+        calls = {"volume_analysis": lambda: L.volume_analysis(),
+                 "volume_averages_synced": lambda: L.volume_averages_synced(0.025),
+                 "spectrum": lambda: L.spectrum(),
+                 "islands": lambda: L.islands(0, thr, as_array=True)}
+        for name, f in calls.items():
+            f()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f()
+            out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+            log("analysis %d^3 %s: %.3f ms" % (N, name, out[name + "_ms"]))
+    return out
 
 
 def batch_leg(pn, steps, out_every=500):
@@ -314,7 +446,7 @@ def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, check_steps=1000, 
     lambda derivative is compared with differences at eps = 1e-6 ... 1e-8,
     which converge to it as eps shrinks."""
     m = model(lam, g)
-    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"]) as L:
+    with lattice(pn, N, m, pn.PN_FP64) as L:
         L.initial_conditions(pn.ic_params(7, a=m["a"], Lambda=m["Lambda"], phi0=m["phi0"]))
         phi, pi = L.get_fields()
     s = float(np.sqrt(m["Lambda"]))
@@ -398,18 +530,18 @@ def cpu_baseline_reference(N, m, target_s=12.0):
         return float(r.stdout.strip().splitlines()[-1])
 
     try:
-        probe = run(4)
-        steps = max(4, int(target_s / max(probe / 4.0, 1e-6)))
+        probe = run(2)
+        steps = max(2, int(target_s / max(probe / 2.0, 1e-6)))
         sec = run(steps)
     except (RuntimeError, ValueError, OSError, subprocess.TimeoutExpired) as e:
         log("reference CPU baseline unavailable: %s" % e)
         return None
     return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": ranks,
             "kind": "reference",
-            "sample": "%d^3 FP64 leapfrog, %d timed steps (%.1f s) of the reference's own evolve() "
-                      "(dynamics.cpp:196-237, halo exchange comm.cpp:63-78) built from its sources with "
-                      "its flags -O3 -msse4.2 -fprefetch-loop-arrays -funroll-all-loops -ffast-math, "
-                      "mpirun -np %d z-slabs" % (N, steps, sec, ranks),
+            "sample": "%d^3 FP64 leapfrog of the workload's model, %d timed steps (%.1f s) of the "
+                      "reference's own evolve() (dynamics.cpp:196-237, halo exchange comm.cpp:63-78) "
+                      "built from its sources with its flags -O3 -msse4.2 -fprefetch-loop-arrays "
+                      "-funroll-all-loops -ffast-math, mpirun -np %d z-slabs" % (N, steps, sec, ranks),
             "host": _host_info(ranks)}
 
 
@@ -417,7 +549,6 @@ def cpu_baseline(N, m, target_s=12.0):
     ref = cpu_baseline_reference(N, m, target_s)
     if ref is not None:
         return ref
-    import ctypes as C
     path = os.path.join(ROOT, "oracle", "libcpu_baseline.so")
     if not os.path.exists(path):
         return None
@@ -427,9 +558,9 @@ def cpu_baseline(N, m, target_s=12.0):
                            C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
     threads = _host_threads()
     ck = C.c_double()
-    probe = lib.cb_run(N, threads, 1, 3, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
+    probe = lib.cb_run(N, threads, 1, 2, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
                        m["kappa"], 0.1, m["phi0"], C.byref(ck))
-    steps = max(3, int(target_s / max(probe / 3.0, 1e-6)))
+    steps = max(2, int(target_s / max(probe / 2.0, 1e-6)))
     sec = lib.cb_run(N, threads, 2, steps, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"],
                      m["kappa"], 0.1, m["phi0"], C.byref(ck))
     return {"value": float(N) ** 3 * steps / sec, "unit": "site-updates/s", "cores": threads,
@@ -440,27 +571,38 @@ def cpu_baseline(N, m, target_s=12.0):
             "host": _host_info(threads)}
 
 
+def extra_rate(pn, N, precision, steps, warmup, m, chunk=None, two_step=True):
+    ms, chunks, used, untimed = run_leg(pn, N, precision, steps, warmup, 1234, m, two_step=two_step,
+                                        chunk=chunk)
+    t = "double" if precision == pn.PN_FP64 else "float"
+    kname, prof = kernel_names(t, used["steps_per_pass"], N)
+    return {"site_updates_per_s": float(N) ** 3 * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+            "steps": steps, "kernel": used, "untimed_steps_before_timing": untimed,
+            "roofline": roofline(kname, N, ms, chunks, used["steps_per_pass"], 8 if t == "double" else 4,
+                                 prof)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=150)
-    ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--out-every", type=int, default=200)
-    ap.add_argument("--batch-steps", type=int, default=5000)
-    ap.add_argument("--precision-size", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=2000, help="timed steps of the 1024^3 workload")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps at least (the kernel choice always completes first)")
+    ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--precision-steps", type=int, default=10000)
     ap.add_argument("--precision-every", type=int, default=1000)
+    ap.add_argument("--no-control", action="store_true", help="no one-ulp FP64 control run")
+    ap.add_argument("--cfg1-steps", type=int, default=10000)
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--kernel", default=None,
+                    help="S:Z fixes the FP64 multi-step kernel (S = 2 or 3 steps per pass, z block Z) "
+                         "instead of the library's timed choice (profiling runs)")
+    ap.add_argument("--analysis", type=int, default=0, help="only time the analysis calls at N^3")
     ap.add_argument("--fp32-only", action="store_true",
                     help="only the FP32 form at --size for --steps steps (for profiling)")
-    ap.add_argument("--precision-only", action="store_true",
-                    help="only the precision form (FP32 against FP64 at --precision-size)")
-    ap.add_argument("--single-step", action="store_true", help="time the one-step kernel only")
-    ap.add_argument("--kernel", default=None,
-                    help="S:Z fixes the multi-step kernel (S = 2 or 3 steps per pass, z block Z) "
-                         "instead of the library's timed choice (profiling runs)")
+    ap.add_argument("--single-step", action="store_true", help="time the one-step FP64 kernel only")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="FP64 workload only")
+    ap.add_argument("--no-extra", action="store_true", help="the workload and its precision form only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (to scale rocprof byte counters)")
     args = ap.parse_args()
@@ -482,74 +624,59 @@ def main():
 
     import phinum_amd as pn
 
-    if args.precision_only:
-        print(json.dumps(precision_leg(pn, args.precision_size, args.precision_steps,
-                                       args.precision_every)), flush=True)
-        return
     N, K, W = args.size, args.steps, args.warmup
-    two = not args.single_step
     kern = tuple(int(v) for v in args.kernel.split(":")) if args.kernel else None
-    m = model()
-    if args.fp32_only:
-        ms32, _, k32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m, kernel=kern)
-        print(json.dumps({"fp32_ms_per_step": ms32 / K, "kernel": k32}), flush=True)
+    if args.analysis:
+        print(json.dumps(analysis_only(pn, args.analysis)), flush=True)
         return
-    log("fp64 %d^3: %d warmup + %d timed steps" % (N, W, K))
+    m = model(lam=1.2, g=0.25, phi0=1.2)
+    if args.fp32_only or args.single_step:
+        prec = pn.PN_FP32 if args.fp32_only else pn.PN_FP64
+        print(json.dumps(extra_rate(pn, N, prec, K, W, m, two_step=not args.single_step)), flush=True)
+        return
+    log("fp64/fp32 %d^3: kernel choice, %d timed steps, on to %d steps" % (N, K, args.precision_steps))
     t0 = time.time()
-    ms64, e64, k64 = run_leg(pn, N, pn.PN_FP64, K, W, args.out_every, 1234, m, two_step=two, kernel=kern)
-    log("fp64 done in %.1fs wall, %.4f ms/step, kernel %s" % (time.time() - t0, ms64 / K, k64))
-    sites = float(N) ** 3
-    value = sites * K / (ms64 * 1e-3)
-    kname, prof = kernel_names("double", k64["steps_per_pass"], N)
+    P = precision_leg(pn, N, K, W, args.precision_steps, args.precision_every,
+                      control=not args.no_control, kernel=kern)
+    log("precision leg done in %.1fs wall" % (time.time() - t0))
+    f64, f32 = P["fp64"], P["fp32"]
+    value = f64["value"]
+    kname, prof = kernel_names("double", f64["kernel"]["steps_per_pass"], N)
+    kname32, prof32 = kernel_names("float", f32["kernel"]["steps_per_pass"], N)
+    tab = P["table"]
     line = {
         "metric": METRIC, "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": ms64 / K, "higher_is_better": True, "scaling": "weak",
+        "warmup": W, "ms_per_step": f64["ms"] / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: seeded initialize==3 vacuum field (seed 1234, phi0=1 in physical units, "
+        "data": "synthetic: seeded initialize==3 vacuum field (seed 1234, phi0=1.2 in physical units, "
                 "modes below k_max/2), generated on the device",
-        "config": {"workload": "fp64 leapfrog %d^3, dx=0.5, dt=0.05, lambda=1, g=0.2, %d steps, "
-                               "energy every %d steps" % (N, K, args.out_every),
-                   "N": N, "out_every": args.out_every, "kernel": k64},
-        "roofline": roofline(kname, N, ms64, K, args.out_every, k64["steps_per_pass"], 8, prof),
-        "energy_drift_fp64": float(np.max(np.abs(e64 - e64[0])) / abs(e64[0])),
+        "config": {"workload": "fp64 leapfrog %d^3 (the precision form's lattice), dx=0.5, dt=0.05, "
+                               "lambda=1.2, g=0.25, %d timed steps" % (N, K),
+                   "N": N, "kernel": f64["kernel"],
+                   "untimed_steps_before_timing": P["untimed_steps_before_timing"],
+                   "kernel_choice_complete_before_timing": True},
+        "roofline": roofline(kname, N, f64["ms"], f64["chunks"], f64["kernel"]["steps_per_pass"], 8, prof),
+        "fp32_comp": {
+            "value": f32["value"], "unit": "site-updates/s", "ms_per_step": f32["ms"] / K,
+            "ratio_vs_fp64": f32["value"] / value, "kernel": f32["kernel"],
+            "steps_with_dE_over_E_le_1e-6": tab["steps_with_dE_over_E_le_1e-6"],
+            "steps_with_synced_dE_over_E_le_1e-6": tab["steps_with_synced_dE_over_E_le_1e-6"],
+            "roofline": roofline(kname32, N, f32["ms"], f32["chunks"], f32["kernel"]["steps_per_pass"], 4,
+                                 prof32)},
+        "precision_%d" % N: tab,
     }
     if not args.no_extra:
-        log("fp64 %d^3 single-step kernel" % N)
-        K1 = min(K, 2000)
-        ms1, _, _ = run_leg(pn, N, pn.PN_FP64, K1, 10, args.out_every, 1234, m, two_step=False)
-        line["single_step_kernel"] = {
-            "value": sites * K1 / (ms1 * 1e-3), "unit": "site-updates/s", "steps": K1,
-            "ms_per_step": ms1 / K1,
-            "roofline": roofline("k_step_vec<double, CoefF64, true, 2, 4, 32, 0>", N, ms1, K1,
-                                 args.out_every, 1, 8, "k_step_vec_fp64_%dcube" % N)}
-        log("fp32 %d^3" % N)
-        ms32, e32, k32 = run_leg(pn, N, pn.PN_FP32, K, W, args.out_every, 1234, m)
-        kname32, prof32 = kernel_names("float", k32["steps_per_pass"], N)
-        v32 = sites * K / (ms32 * 1e-3)
-        rel = np.abs(e32 - e64) / np.abs(e64)
-        over = np.nonzero(rel > 1e-6)[0]
-        line["fp32_comp"] = {
-            "value": v32, "unit": "site-updates/s", "ms_per_step": ms32 / K,
-            "ratio_vs_fp64": v32 / value,
-            "dE_over_E_vs_fp64_max": float(np.max(rel)),
-            # outputs are every out_every steps after the warmup; the two
-            # chaotic trajectories separate roughly exponentially
-            "steps_with_dE_over_E_le_1e-6": int(K if over.size == 0 else (over[0] - 1) * args.out_every),
-            "dE_over_E_vs_fp64_by_output": [float("%.3g" % r) for r in rel[::max(1, len(rel) // 10)]],
-            "energy_drift_fp32": float(np.max(np.abs(e32 - e32[0])) / abs(e32[0])),
-            "kernel": k32,
-            "roofline": roofline(kname32, N, ms32, K, args.out_every, k32["steps_per_pass"], 4, prof32),
-        }
-        N5 = 512
-        log("fp64 %d^3" % N5)
-        ms5, _, k5 = run_leg(pn, N5, pn.PN_FP64, 200, 150, 200, 5, m)
-        kname5, prof5 = kernel_names("double", k5["steps_per_pass"], N5)
-        line["fp64_512"] = {"ms_per_step": ms5 / 200, "kernel": k5,
-                            "site_updates_per_s": float(N5) ** 3 * 200 / (ms5 * 1e-3),
-                            "roofline": roofline(kname5, N5, ms5, 200, 200, k5["steps_per_pass"], 8, prof5)}
-        log("precision form %d^3" % args.precision_size)
-        line["precision_%d" % args.precision_size] = precision_leg(
-            pn, args.precision_size, args.precision_steps, args.precision_every)
+        mc = model()
+        for n5 in (256, 512):
+            log("fp64 and fp32 %d^3" % n5)
+            line["fp64_%d" % n5] = extra_rate(pn, n5, pn.PN_FP64, 2000, 10, mc)
+            line["fp32_%d" % n5] = extra_rate(pn, n5, pn.PN_FP32, 2000, 10, mc)
+            line["fp32_%d" % n5]["ratio_vs_fp64"] = (line["fp32_%d" % n5]["site_updates_per_s"] /
+                                                    line["fp64_%d" % n5]["site_updates_per_s"])
+        log("fp64 256^3 one-step kernel")
+        line["single_step_kernel_256"] = extra_rate(pn, 256, pn.PN_FP64, 1000, 10, mc, two_step=False)
+        log("the reference's loop with outputs, 256^3")
+        line["cfg1_loop_256"] = cfg1_leg(pn, 256, args.cfg1_steps)
         log("batch of 64 realizations")
         line["batch_64"] = batch_leg(pn, args.batch_steps)
         log("adjoint 128^3")

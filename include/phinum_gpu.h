@@ -218,6 +218,13 @@ int pn_extrema_get(pn_ctx *c, pn_extrema *out);
  * (fieldanalysis.cpp:748-799) calls them back to back; either output may be
  * NULL.  The values are those of the two separate calls. */
 int pn_volume_analysis(pn_ctx *c, pn_averages *avg, pn_extrema *ext);
+/* The volume averages with Pi brought to Phi's time by a kick of half_dt
+ * (updateMomenta, dynamics.cpp:52-175, into scratch; the fields are
+ * unchanged): between leapfrog steps Pi lags Phi by half a step, so Energy of
+ * pn_volume_averages mixes phi(t) with pi(t - dt/2) (fieldanalysis.cpp:625).
+ * half_dt = dtau/2 gives the time-synchronised energy.  No reference
+ * counterpart; not available on a batch view. */
+int pn_volume_averages_synced(pn_ctx *c, double half_dt, pn_averages *out);
 /* pn_volume_analysis without allocating (no reference counterpart): for
  * callers that cannot call hipMalloc, such as code under HIP graph capture or
  * a serving loop.  phi and pi are the caller's dense device fields of N^3

@@ -138,6 +138,7 @@ def lib():
         "pn_volume_averages": (C.c_int, [C.c_void_p, P(PnAverages)]),
         "pn_extrema_get": (C.c_int, [C.c_void_p, P(PnExtrema)]),
         "pn_volume_analysis": (C.c_int, [C.c_void_p, P(PnAverages), P(PnExtrema)]),
+        "pn_volume_averages_synced": (C.c_int, [C.c_void_p, C.c_double, P(PnAverages)]),
         "pn_volume_analysis_noalloc": (C.c_int, [P(PnParams), C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p]),
         "pn_spectrum_nbins": (C.c_int, [C.c_int, C.c_double]),
@@ -323,6 +324,13 @@ class Lattice:
         avg, ext = PnAverages(), PnExtrema()
         self._chk(self.L.pn_volume_analysis(self.h, C.byref(avg), C.byref(ext)))
         return avg, ext
+
+    def volume_averages_synced(self, half_dt):
+        """Averages with Pi advanced by a kick of half_dt to Phi's time (the
+        time-synchronised energy; pn_volume_averages_synced)."""
+        out = PnAverages()
+        self._chk(self.L.pn_volume_averages_synced(self.h, float(half_dt), C.byref(out)))
+        return out
 
     def spectrum(self):
         nb = self.L.pn_spectrum_nbins(self.N, self.a)

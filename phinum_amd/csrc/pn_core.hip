@@ -75,7 +75,7 @@ static inline StepGeom step_geom(int N, int by, int zb) {
 // carrying phi(z-1), phi(z), phi(z+1) in registers.  In-plane neighbours are
 // read from the same or adjacent cache lines (x) or from rows the tile's other
 // waves have just brought into L1/L2 (y).  MODE 0: fused step (writes phin);
-// MODE 1: kick only (Pi in place).
+// MODE 1: kick only (Pi in place); MODE 2: kick into phin (Pi unchanged).
 template <typename T, typename C, bool POW2, int BY, int ZB, int MODE>
 __global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *__restrict__ pi,
                                                   T *__restrict__ phin, StepGeom g, C k, T dtk,
@@ -114,7 +114,11 @@ __global__ __launch_bounds__(64 * BY) void k_step(const T *__restrict__ phi, T *
     } else {
       site_update_f32(k, vxp, vxm, vyp, vym, fp, fm, fc, p, dtk);
     }
-    __builtin_nontemporal_store(p, &pi[base + row + x]);
+    if constexpr (MODE == 2) {
+      __builtin_nontemporal_store(p, &phin[base + row + x]);  // the kicked Pi, elsewhere
+    } else {
+      __builtin_nontemporal_store(p, &pi[base + row + x]);
+    }
     if constexpr (MODE == 0) {
       const T fnew = fc + p * dtd;
       __builtin_nontemporal_store(fnew, &phin[base + row + x]);
@@ -187,7 +191,11 @@ __global__ __launch_bounds__(64 * BY) void k_step_vec(const T *__restrict__ phi,
         p[j] = pj;
       }
     }
-    __builtin_nontemporal_store(p, (V *)&pi[base + row + x]);
+    if constexpr (MODE == 2) {
+      __builtin_nontemporal_store(p, (V *)&phin[base + row + x]);  // the kicked Pi, elsewhere
+    } else {
+      __builtin_nontemporal_store(p, (V *)&pi[base + row + x]);
+    }
     if constexpr (MODE == 0) {
       V fnew;
 #pragma unroll
@@ -316,15 +324,18 @@ static int launch_step(pn_ctx *c, int mode, double dtk, double dtd) {
     CoefF64 k = coef64(c);
     if (c->k.a2_pow2) {
       if (mode == 0) launch_step_t<double, CoefF64, true, 0>(c, k, dtk, dtd);
-      else launch_step_t<double, CoefF64, true, 1>(c, k, dtk, dtd);
+      else if (mode == 1) launch_step_t<double, CoefF64, true, 1>(c, k, dtk, dtd);
+      else launch_step_t<double, CoefF64, true, 2>(c, k, dtk, dtd);
     } else {
       if (mode == 0) launch_step_t<double, CoefF64, false, 0>(c, k, dtk, dtd);
-      else launch_step_t<double, CoefF64, false, 1>(c, k, dtk, dtd);
+      else if (mode == 1) launch_step_t<double, CoefF64, false, 1>(c, k, dtk, dtd);
+      else launch_step_t<double, CoefF64, false, 2>(c, k, dtk, dtd);
     }
   } else {
     CoefF32 k = coef32(c);
     if (mode == 0) launch_step_t<float, CoefF32, true, 0>(c, k, (float)dtk, (float)dtd);
-    else launch_step_t<float, CoefF32, true, 1>(c, k, (float)dtk, (float)dtd);
+    else if (mode == 1) launch_step_t<float, CoefF32, true, 1>(c, k, (float)dtk, (float)dtd);
+    else launch_step_t<float, CoefF32, true, 2>(c, k, (float)dtk, (float)dtd);
   }
   PN_CHECK_LAUNCH(c);
   if (mode == 0) c->cur ^= 1;
@@ -1536,6 +1547,29 @@ extern "C" int pn_extrema_get(pn_ctx *c, pn_extrema *out) {
   int rc = run_volume(c);
   if (rc != PN_OK) return rc;
   vol_unpack(c->host_pin, nullptr, out);
+  return PN_OK;
+}
+
+// Averages with Pi brought to Phi's time: Pi lives half a step behind Phi
+// between leapfrog steps (the first kick is a half kick, dynamics.cpp:196-213),
+// so Energy mixes phi(t) with pi(t - dt/2).  The kick of updateMomenta
+// (dynamics.cpp:52-175) with factor half_dt, written into the idle Phi
+// buffer, gives pi(t) to the leapfrog's order, and the volume sums run on
+// (Phi, that buffer).  No reference counterpart: the precision form's
+// time-synchronised energy.  The fields are unchanged.
+extern "C" int pn_volume_averages_synced(pn_ctx *c, double half_dt, pn_averages *out) {
+  if (!c || !out) return pn_fail(c, PN_ERR_ARG, "pn_volume_averages_synced: NULL argument");
+  if (c->is_view)
+    return pn_fail(c, PN_ERR_STATE, "pn_volume_averages_synced: not available on a batch view");
+  PN_HIP(c, hipSetDevice(c->p.device));
+  int rc = launch_step(c, 2, half_dt, 0.0);  // phi[cur ^ 1] = Pi kicked by half_dt
+  if (rc != PN_OK) return rc;
+  PN_HIP(c, launch_volume(c->p, c->k.a2_pow2, c->phi[c->cur], c->phi[c->cur ^ 1], c->stream,
+                          vol_scratch(c), c->red_out));
+  PN_HIP(c, hipMemcpyAsync(c->host_pin, c->red_out, kVolOut * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  vol_unpack(c->host_pin, out, nullptr);
   return PN_OK;
 }
 

@@ -456,6 +456,36 @@ def test_extrema_of_signed_zeros():
     assert (ext.minpos, ext.maxpos, ext.maxabspos) == tuple(int(i) for i in idx)
 
 
+@pytest.mark.parametrize("N,steps", [(16, 7), (33, 4)])
+def test_synced_energy_is_the_half_kicked_averages(N, steps):
+    # pn_volume_averages_synced: the averages of (phi, pi + half kick), the
+    # kick of updateMomenta with factor dtau/2 (dynamics.cpp:52-175); the
+    # oracle's kick on a copy of pi, then its averages; the fields unchanged
+    # and the next steps the same bits as without the call
+    import ctypes as C
+    m = O.model(N)
+    phi0, pi0 = _fields(N, 90 + N)
+    dtk, dtd = pn.time_factors(0.0, 0.1, m["a"], 2 * steps)[:2]
+    half = 0.5 * 0.1 * m["a"]
+    with _lat(m) as L:
+        L.set_fields(phi0, pi0)
+        L.evolve(dtk[:steps], dtd[:steps])
+        f, p = L.get_fields()
+        syn = L.volume_averages_synced(half).as_array()
+        f2, p2 = L.get_fields()
+        L.evolve(dtk[steps:], dtd[steps:])
+        fa, pa = L.get_fields()
+    assert np.array_equal(_bits(f), _bits(f2)) and np.array_equal(_bits(p), _bits(p2))
+    phi = O.to_ref(f, N)
+    pik = O.to_ref(p, N)
+    prm = O.params(m)
+    O.lib().or_update_momenta(C.byref(prm), O.dptr(phi), O.dptr(pik), half)
+    want = O.volume_averages(m, f, O.from_ref(pik, N))
+    np.testing.assert_allclose(syn, want, rtol=1e-13, atol=1e-15)
+    ef, ep = O.evolve_tables(m, f, p, dtk[steps:], dtd[steps:])
+    assert np.array_equal(_bits(fa), _bits(ef)) and np.array_equal(_bits(pa), _bits(ep))
+
+
 def test_extrema_ties_pick_first_site():
     N = 8
     m = O.model(N)
