@@ -568,6 +568,8 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
   for (int i = 0; i < 2; i++)
     if (hipMemsetAsync(b->phi[i], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
   if (hipMemsetAsync(b->pib[0], 0, bytes, b->stream) != hipSuccess) return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
+  if (b->pib[1] && hipMemsetAsync(b->pib[1], 0, bytes, b->stream) != hipSuccess)
+    return fail(PN_ERR_HIP, "pn_batch_create: zero fill failed");
   b->hdesc.assign(R, BatchDesc());
   if (upload_desc(b) != PN_OK) return fail(PN_ERR_HIP, "pn_batch_create: descriptor upload failed");
   for (int r = 0; r < R; r++) {
@@ -832,6 +834,17 @@ extern "C" int pn_batch_autotune(pn_batch *b, int trial_steps, int *out, int *nc
     results.clear();
     total = 0;
     for (int N : classes) {
+      // every class is timed on the caller's fields, not on what the previous
+      // class's trials left in the buffers
+      b->cur = saved_cur;
+      b->pcur = saved_pcur;
+      hipError_t er = hipMemcpyAsync(b->phi[saved_cur], save_phi, bytes, hipMemcpyDeviceToDevice, b->stream);
+      if (er == hipSuccess)
+        er = hipMemcpyAsync(b->pib[saved_pcur], save_pi, bytes, hipMemcpyDeviceToDevice, b->stream);
+      if (er != hipSuccess) {
+        rc = bfail(b, PN_ERR_HIP, "pn_batch_autotune: ", hipGetErrorString(er));
+        return;
+      }
       std::vector<int> mask(b->R, 0);
       for (int r = 0; r < b->R; r++) mask[r] = b->re[r].N == N;
       double best = 1e30;
