@@ -92,6 +92,12 @@ static inline long long pn_ref_pos(const pn_ctx *c, long long dense) {
   return dense + (long long)c->p.N * c->p.N;
 }
 
+// The peak of island `id` of the context's last pn_islands call as the
+// reference's breadth-first search finds it from site seed_pos (pos()
+// numbering): the first site holding the island's largest key in that
+// order (islands.cpp:90-279).  Positions in pos() numbering.
+int pn_islands_seeded_peak(pn_ctx *c, int id, long long seed_pos, long long *peak_pos, double *peak_value);
+
 // shared by several translation units
 int pn_reduce_blocks(const pn_ctx *c);
 void pn_free_spectrum(pn_ctx *c);

@@ -27,10 +27,16 @@
 
 #include <algorithm>
 #include <cmath>
+#include <new>
 #include <vector>
 
 #include "pn_internal.h"
 #include "pn_kernels.h"
+
+struct IslCoef {
+  double a2, cm, lmode, d6, thr, thr2h;
+  int mode;
+};
 
 struct pn_island_state {
   int *label = nullptr;      // N^3
@@ -55,9 +61,23 @@ struct pn_island_state {
   long long ccap = 0;                  // allocated chunks (large islands: half as many)
   int *chunk_isl = nullptr, *chunk_rank = nullptr, *big_isl = nullptr, *big_c0 = nullptr;
   double *cpart = nullptr;             // per chunk: (hi, lo) of the four sums
-  // host staging of the per-island integers and sums, kept across calls
-  std::vector<long long> h_ii;
-  std::vector<double> h_dd;
+  // catalog rows assembled on the device, and the label array handed to the
+  // caller, kept across calls (grown on demand)
+  pn_island *rows = nullptr;
+  long long rcap = 0;
+  int *dlab = nullptr;
+  // pn_islands_lookup's positions and answers (grown on demand)
+  long long *lpos = nullptr;
+  int *lout = nullptr;
+  long long lcap = 0;
+  // one island's keys and peak values for the breadth-first order of tied peaks
+  double *kv = nullptr;
+  long long kvcap = 0;
+  // the last call: its coefficients, its island count and the starts of the
+  // islands among the sorted sites (host copy, read on demand)
+  IslCoef last{};
+  int nisl = 0;
+  int lgN = -1;
 };
 
 void pn_free_islands(pn_ctx *c) {
@@ -83,6 +103,11 @@ void pn_free_islands(pn_ctx *c) {
   hipFree(s->big_isl);
   hipFree(s->big_c0);
   hipFree(s->cpart);
+  hipFree(s->rows);
+  hipFree(s->dlab);
+  hipFree(s->lpos);
+  hipFree(s->lout);
+  hipFree(s->kv);
   delete s;
   c->isl = nullptr;
 }
@@ -92,10 +117,6 @@ static constexpr int kChunk = kT * 16;  // sites per block in the numbering pass
 static constexpr int kNI = 7;           // integer stats per island
 static constexpr int kND = 5;           // double stats per island (+ peak value)
 
-struct IslCoef {
-  double a2, cm, lmode, d6, thr, thr2h;
-  int mode;
-};
 
 // site energy density Delta (fieldanalysis.cpp:447,514-515)
 template <typename T>
@@ -126,14 +147,36 @@ __device__ __forceinline__ bool site_mask(const T *phi, const T *pi, long long i
   return d > k.thr;  // islands.cpp:83
 }
 
+// x, y, z of site i (< 2^31): shifts when N is a power of two (lgN >= 0)
+__device__ __forceinline__ void site_xyz(int i, int N, int lgN, int &x, int &y, int &z) {
+  if (lgN >= 0) {
+    x = i & (N - 1);
+    y = (i >> lgN) & (N - 1);
+    z = i >> (2 * lgN);
+  } else {
+    const int r = i / N;
+    x = i - r * N;
+    y = r % N;
+    z = r / N;
+  }
+}
+
+// mode 0 reads only the site itself; mode 1 needs its position (32-bit, and
+// shifts for power-of-two N)
 template <typename T>
 __global__ __launch_bounds__(kT) void k_isl_init(const T *__restrict__ phi, const T *__restrict__ pi,
-                                                 int N, IslCoef k, int *__restrict__ label) {
-  const long long n3 = (long long)N * N * N, P = (long long)N * N;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3;
-       i += (long long)gridDim.x * kT) {
-    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    label[i] = site_mask(phi, pi, i, x, y, z, N, k, nullptr) ? (int)i : -1;
+                                                 int N, int lgN, IslCoef k, int *__restrict__ label) {
+  const int n3 = N * N * N;  // below 2^31 (pn_islands checks)
+  for (int i = (int)blockIdx.x * kT + (int)threadIdx.x; i < n3; i += (int)gridDim.x * kT) {
+    bool in;
+    if (k.mode == 0) {
+      in = site_mask(phi, pi, i, 0, 0, 0, N, k, nullptr);
+    } else {
+      int x, y, z;
+      site_xyz(i, N, lgN, x, y, z);
+      in = site_mask(phi, pi, i, x, y, z, N, k, nullptr);
+    }
+    label[i] = in ? i : -1;
   }
 }
 
@@ -327,20 +370,6 @@ __global__ void k_isl_starts(const int *__restrict__ key, int m, int nisl, int *
   if (p >= m) return;
   if (p == 0 || key[p - 1] != key[p]) ist[key[p]] = p;
   if (p == m - 1) ist[nisl] = m;
-}
-
-// x, y, z of site i (< 2^31): shifts when N is a power of two (lgN >= 0)
-__device__ __forceinline__ void site_xyz(int i, int N, int lgN, int &x, int &y, int &z) {
-  if (lgN >= 0) {
-    x = i & (N - 1);
-    y = (i >> lgN) & (N - 1);
-    z = i >> (2 * lgN);
-  } else {
-    const int r = i / N;
-    x = i - r * N;
-    y = r % N;
-    z = r / N;
-  }
 }
 
 // the peak key of a site: phi^2 (mode 0) or Delta (mode 1)
@@ -672,6 +701,185 @@ __global__ void k_isl_peakval(const T *__restrict__ phi, const T *__restrict__ p
       k.mode == 0 ? (double)phi[pk] : site_delta(phi, pi, (long long)pk, x, y, z, N, k);
 }
 
+// centroid of a row from its peak and offset sums: x and y wrapped onto the
+// lattice, z plain (the same expressions on the device and on the host)
+__host__ __device__ inline void isl_centroid(pn_island &I, int N) {
+  const long long P = (long long)N * N;
+  const long long pk = I.peak_pos - P;
+  const double px = (double)(pk % N), py = (double)((pk / N) % N), pz = (double)(pk / P);
+  const double NN = (double)I.nn;
+  double cx = px + (double)I.sum_dx / NN, cy = py + (double)I.sum_dy / NN;
+  cx = fmod(cx + N, (double)N);
+  cy = fmod(cy + N, (double)N);
+  I.cx = cx;
+  I.cy = cy;
+  I.cz = pz + (double)I.sum_dz / NN;
+}
+
+// catalog rows as BFS finishes them (islands.cpp:240-245): volume, averages
+// and energies times a^3, positions in the reference's pos() numbering
+__global__ void k_isl_rows(int n, const long long *__restrict__ iint, const double *__restrict__ idbl,
+                           int N, int lgN, double a, pn_island *__restrict__ rows) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  (void)lgN;
+  const long long *q = iint + (long long)id * kNI;
+  const double *d = idbl + (long long)id * kND;
+  const long long P = (long long)N * N;
+  const double a3 = (a * a) * a;
+  pn_island I;
+  const long long NN = q[0];
+  I.index = id;
+  I.nn = NN;
+  I.first_pos = q[1] + P;
+  I.peak_pos = q[2] + P;
+  I.peak_ties = q[3] - 1;
+  I.volume = NN * a * a * a;
+  I.avg_phi = d[0] / (double)(NN);
+  I.pot_en = d[1] * a3;
+  I.kin_en = d[2] * a3;
+  I.grad_en = d[3] * a3;
+  I.sum_dx = q[4];
+  I.sum_dy = q[5];
+  I.sum_dz = q[6];
+  I.peak_value = d[4];
+  isl_centroid(I, N);
+  rows[id] = I;
+}
+
+// the peak key (phi^2 or Delta) and the peak value (phi or Delta) of the
+// given sites, as the catalog kernels compute them (isl_key, k_isl_peakval)
+template <typename T>
+__global__ void k_isl_keyvals(const T *__restrict__ phi, const T *__restrict__ pi, int N, int lgN,
+                              IslCoef k, const int *__restrict__ site, int n, double *__restrict__ kv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int i = site[j];
+  if (k.mode == 0) {
+    const double f = (double)phi[i];
+    kv[2 * j] = f * f;
+    kv[2 * j + 1] = f;
+  } else {
+    int x, y, z;
+    site_xyz(i, N, lgN, x, y, z);
+    const double d = site_delta(phi, pi, (long long)i, x, y, z, N, k);
+    kv[2 * j] = d;
+    kv[2 * j + 1] = d;
+  }
+}
+
+// The peak of island `id` of the last pn_islands call as the reference's BFS
+// finds it from site `seed` (dense index): the queue order of islands.cpp:
+// 90-279 (x+1, x-1, y+1, y-1 periodic, then z+1 and z-1 inside the lattice)
+// and the first strict maximum of the key.  Only islands with equal maxima
+// need it (a unique maximum does not depend on the order); it runs on the
+// host over the island's sites (sorted, so membership is a binary search) and
+// stops at the first site holding the island's largest key.  Fills
+// peak_pos, peak_value, the offset sums and the centroid of *row.
+template <typename T>
+static int isl_bfs_peak(pn_ctx *c, int id, long long seed, pn_island *row) {
+  pn_island_state *s = c->isl;
+  const int N = c->p.N;
+  const long long P = (long long)N * N;
+  if (!s || id < 0 || id >= s->nisl) return pn_fail(c, PN_ERR_STATE, "pn_islands: no island %d", id);
+  int se[2];
+  PN_HIP(c, hipMemcpyAsync(se, s->ist + id, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  const int n = se[1] - se[0];
+  std::vector<int> sites;
+  std::vector<double> kv;
+  std::vector<int> queue;
+  std::vector<char> seen;
+  try {
+    sites.resize(n);
+    kv.resize(2 * (size_t)n);
+    queue.reserve(n);
+    seen.assign(n, 0);
+  } catch (const std::bad_alloc &) {
+    return pn_fail(c, PN_ERR_NOMEM, "pn_islands: no host memory for an island of %d sites", n);
+  }
+  if (2LL * n > s->kvcap) {
+    hipFree(s->kv);
+    s->kv = nullptr;
+    s->kvcap = 0;
+    PN_HIP(c, hipMalloc(&s->kv, sizeof(double) * 2 * (size_t)n));
+    s->kvcap = 2LL * n;
+  }
+  hipLaunchKernelGGL(k_isl_keyvals<T>, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                     (const T *)c->phi[c->cur], (const T *)c->pi, N, s->lgN, s->last, s->msite2 + se[0],
+                     n, s->kv);
+  PN_CHECK_LAUNCH(c);
+  PN_HIP(c, hipMemcpyAsync(sites.data(), s->msite2 + se[0], sizeof(int) * (size_t)n, hipMemcpyDeviceToHost,
+                           c->stream));
+  PN_HIP(c, hipMemcpyAsync(kv.data(), s->kv, sizeof(double) * 2 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  double M = kv[0];
+  for (int j = 1; j < n; j++) M = kv[2 * j] > M ? kv[2 * j] : M;
+  auto find = [&](long long site) -> int {
+    auto it = std::lower_bound(sites.begin(), sites.end(), (int)site);
+    return (it != sites.end() && *it == (int)site) ? (int)(it - sites.begin()) : -1;
+  };
+  const int s0 = find(seed);
+  if (s0 < 0) return pn_fail(c, PN_ERR_STATE, "pn_islands: site %lld is not in island %d", seed, id);
+  int best = s0;
+  queue.push_back(s0);
+  seen[s0] = 1;
+  for (size_t head = 0; head < queue.size() && !(kv[2 * best] == M); head++) {
+    const int u = queue[head];
+    if (kv[2 * u] > kv[2 * best]) best = u;  // islands.cpp:110-116
+    if (kv[2 * best] == M) break;
+    const long long i = sites[u];
+    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
+    const long long row0 = i - x, plane0 = i - (long long)y * N - x;
+    const long long nb[6] = {row0 + (x + 1 == N ? 0 : x + 1), row0 + (x == 0 ? N - 1 : x - 1),
+                             plane0 + (long long)(y + 1 == N ? 0 : y + 1) * N + x,
+                             plane0 + (long long)(y == 0 ? N - 1 : y - 1) * N + x,
+                             z + 1 < N ? i + P : -1, z > 0 ? i - P : -1};
+    for (int j = 0; j < 6; j++) {
+      if (nb[j] < 0) continue;
+      const int v = find(nb[j]);
+      if (v >= 0 && !seen[v]) {
+        seen[v] = 1;
+        queue.push_back(v);
+      }
+    }
+  }
+  const long long pk = sites[best];
+  const int px = (int)(pk % N), py = (int)((pk / N) % N), pz = (int)(pk / P);
+  long long sx = 0, sy = 0, sz = 0;
+  for (int j = 0; j < n; j++) {
+    const long long i = sites[j];
+    int dx = (int)(i % N) - px, dy = (int)((i / N) % N) - py;
+    if (dx >= N - N / 2) dx -= N;
+    if (dx < -(N / 2)) dx += N;
+    if (dy >= N - N / 2) dy -= N;
+    if (dy < -(N / 2)) dy += N;
+    sx += dx;
+    sy += dy;
+    sz += (int)(i / P) - pz;
+  }
+  row->peak_pos = pk + P;
+  row->peak_value = kv[2 * best + 1];
+  row->sum_dx = sx;
+  row->sum_dy = sy;
+  row->sum_dz = sz;
+  isl_centroid(*row, N);
+  return PN_OK;
+}
+
+int pn_islands_seeded_peak(pn_ctx *c, int id, long long seed_pos, long long *peak_pos, double *peak_value) {
+  if (!c || !c->isl) return pn_fail(c, PN_ERR_STATE, "pn_islands_seeded_peak: no pn_islands call yet");
+  pn_island row{};
+  row.nn = 1;
+  const long long seed = seed_pos - (long long)c->p.N * c->p.N;
+  const int rc = c->p.precision == PN_FP64 ? isl_bfs_peak<double>(c, id, seed, &row)
+                                           : isl_bfs_peak<float>(c, id, seed, &row);
+  if (rc != PN_OK) return rc;
+  *peak_pos = row.peak_pos;
+  *peak_value = row.peak_value;
+  return PN_OK;
+}
+
 template <typename T>
 static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
   pn_island_state *s = c->isl;
@@ -680,7 +888,15 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
   const int grid = (int)std::min<long long>(8192, (n3 + kT - 1) / kT);
   const T *phi = (const T *)c->phi[c->cur];
   const T *pi = (const T *)c->pi;
-  hipLaunchKernelGGL(k_isl_init<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, k, s->label);
+  int lgN = -1;
+  if ((N & (N - 1)) == 0) {
+    lgN = 0;
+    while ((1 << lgN) < N) lgN++;
+  }
+  s->last = k;
+  s->lgN = lgN;
+  s->nisl = 0;
+  hipLaunchKernelGGL(k_isl_init<T>, dim3(grid), dim3(kT), 0, c->stream, phi, pi, N, lgN, k, s->label);
   hipLaunchKernelGGL(k_isl_merge, dim3(grid), dim3(kT), 0, c->stream, s->label, N);
   hipLaunchKernelGGL(k_isl_flatten, dim3(grid), dim3(kT), 0, c->stream, s->label, n3);
   hipLaunchKernelGGL(k_isl_count, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3, s->blk);
@@ -754,11 +970,6 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
       PN_HIP(c, hipMalloc(&s->cpart, (size_t)ccap * 8 * sizeof(double)));
       s->ccap = ccap;
     }
-    int lgN = -1;
-    if ((N & (N - 1)) == 0) {
-      lgN = 0;
-      while ((1 << lgN) < N) lgN++;
-    }
     hipLaunchKernelGGL(k_isl_prep, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, nisl,
                        s->ikey, s->iint, s->idbl);
     hipLaunchKernelGGL(k_isl_mscatter, dim3((unsigned)s->nblk), dim3(kT), 0, c->stream, s->label, n3,
@@ -785,58 +996,38 @@ static int run_islands(pn_ctx *c, const IslCoef &k, pn_catalog *out) {
     hipLaunchKernelGGL(k_isl_peakval<T>, dim3((nisl + 255) / 256), dim3(256), 0, c->stream, phi, pi,
                        N, lgN, k, nisl, s->iint, s->idbl);
   }
-  int *dlab = nullptr;
-  if (out->labels) PN_HIP(c, hipMalloc(&dlab, (size_t)n3 * sizeof(int)));
-  if (dlab)
-    hipLaunchKernelGGL(k_isl_labels, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, dlab);
-  hipError_t e = hipGetLastError();
-  if ((size_t)nisl * kNI > s->h_ii.size()) {
-    s->h_ii.resize((size_t)nisl * kNI + (size_t)nisl / 4 * kNI);
-    s->h_dd.resize((size_t)nisl * kND + (size_t)nisl / 4 * kND);
-  }
-  const long long *ii = s->h_ii.data();
-  const double *dd = s->h_dd.data();
-  if (e == hipSuccess && nisl > 0)
-    e = hipMemcpyAsync(s->h_ii.data(), s->iint, (size_t)nisl * kNI * sizeof(long long), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess && nisl > 0)
-    e = hipMemcpyAsync(s->h_dd.data(), s->idbl, (size_t)nisl * kND * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess && dlab)
-    e = hipMemcpyAsync(out->labels, dlab, (size_t)n3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (dlab) hipFree(dlab);
-  if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands: %s", hipGetErrorString(e));
-
-  // finalise on the host exactly as BFS does (islands.cpp:240-245)
-  const double a = c->p.a, a3 = (a * a) * a;
-  const long long P = (long long)N * N;
+  s->nisl = nisl;
+  // the rows, assembled on the device (isl_row) and copied once into the
+  // caller's array
   const int ncopy = std::min(nisl, out->capacity);
+  if (ncopy > s->rcap) {
+    hipFree(s->rows);
+    s->rows = nullptr;
+    s->rcap = 0;
+    const long long rc = std::max<long long>(ncopy + ncopy / 4, 1024);
+    PN_HIP(c, hipMalloc(&s->rows, (size_t)rc * sizeof(pn_island)));
+    s->rcap = rc;
+  }
+  if (out->labels && !s->dlab) PN_HIP(c, hipMalloc(&s->dlab, (size_t)n3 * sizeof(int)));
+  if (out->labels)
+    hipLaunchKernelGGL(k_isl_labels, dim3(grid), dim3(kT), 0, c->stream, s->label, n3, s->cid, s->dlab);
+  if (ncopy > 0)
+    hipLaunchKernelGGL(k_isl_rows, dim3((ncopy + 255) / 256), dim3(256), 0, c->stream, ncopy, s->iint,
+                       s->idbl, N, lgN, c->p.a, s->rows);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && ncopy > 0)
+    e = hipMemcpyAsync(out->islands, s->rows, (size_t)ncopy * sizeof(pn_island), hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess && out->labels)
+    e = hipMemcpyAsync(out->labels, s->dlab, (size_t)n3 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands: %s", hipGetErrorString(e));
+  // equal maxima: the reference keeps the first of them in its breadth-first
+  // order from the island's first site (islands.cpp:104-116,200-236)
   for (int id = 0; id < ncopy; id++) {
-    const long long *q = &ii[(size_t)id * kNI];
-    const double *d = &dd[(size_t)id * kND];
-    pn_island &I = out->islands[id];
-    const long long NN = q[0];
-    I.index = id;
-    I.nn = NN;
-    I.first_pos = pn_ref_pos(c, q[1]);
-    I.peak_pos = pn_ref_pos(c, q[2]);
-    I.peak_ties = q[3] - 1;
-    I.volume = NN * a * a * a;
-    I.avg_phi = d[0] / (double)(NN);
-    I.pot_en = d[1] * a3;
-    I.kin_en = d[2] * a3;
-    I.grad_en = d[3] * a3;
-    I.sum_dx = q[4];
-    I.sum_dy = q[5];
-    I.sum_dz = q[6];
-    const long long pk = q[2];
-    const double px = (double)(pk % N), py = (double)((pk / N) % N), pz = (double)(pk / P);
-    double cx = px + (double)q[4] / (double)NN, cy = py + (double)q[5] / (double)NN;
-    cx = std::fmod(cx + N, (double)N);
-    cy = std::fmod(cy + N, (double)N);
-    I.cx = cx;
-    I.cy = cy;
-    I.cz = pz + (double)q[6] / (double)NN;
-    I.peak_value = d[4];
+    if (out->islands[id].peak_ties == 0) continue;
+    int rc = isl_bfs_peak<T>(c, id, out->islands[id].first_pos - (long long)N * N, &out->islands[id]);
+    if (rc != PN_OK) return rc;
   }
   if (nisl > out->capacity)
     return pn_fail(c, PN_ERR_CAPACITY, "pn_islands: %d islands, capacity %d", nisl, out->capacity);
@@ -850,7 +1041,8 @@ extern "C" int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out) {
   if (c->n3 > 0x7fffffffLL) return pn_fail(c, PN_ERR_ARG, "pn_islands: lattice too large for int labels");
   PN_HIP(c, hipSetDevice(c->p.device));
   if (!c->isl) {
-    c->isl = new pn_island_state();
+    c->isl = new (std::nothrow) pn_island_state();
+    if (!c->isl) return pn_fail(c, PN_ERR_NOMEM, "pn_islands: out of host memory");
     pn_island_state *s = c->isl;
     s->nblk = (c->n3 + kChunk - 1) / kChunk;
     PN_HIP(c, hipMalloc(&s->label, (size_t)c->n3 * sizeof(int)));
@@ -887,28 +1079,39 @@ extern "C" int pn_islands_lookup(pn_ctx *c, int n, const long long *pos, int *is
     return pn_fail(c, PN_ERR_ARG, "pn_islands_lookup: bad argument");
   if (!c->isl) return pn_fail(c, PN_ERR_STATE, "pn_islands_lookup: no pn_islands call yet");
   if (n == 0) return PN_OK;
+  pn_island_state *s = c->isl;
   const long long P = (long long)c->p.N * c->p.N;
-  std::vector<long long> dense(n);
+  std::vector<long long> dense;
+  try {
+    dense.resize(n);
+  } catch (const std::bad_alloc &) {
+    return pn_fail(c, PN_ERR_NOMEM, "pn_islands_lookup: no host memory for %d positions", n);
+  }
   for (int i = 0; i < n; i++) {
     dense[i] = pos[i] - P;  // reference pos() (one z halo plane below) -> dense
     if (dense[i] < 0 || dense[i] >= c->n3)
       return pn_fail(c, PN_ERR_ARG, "pn_islands_lookup: pos %lld is not an interior site", pos[i]);
   }
   PN_HIP(c, hipSetDevice(c->p.device));
-  long long *dd = nullptr;
-  int *dout = nullptr;
-  PN_HIP(c, hipMalloc(&dd, sizeof(long long) * n));
-  hipError_t e = hipMalloc(&dout, sizeof(int) * n);
-  if (e == hipSuccess) e = hipMemcpyAsync(dd, dense.data(), sizeof(long long) * n, hipMemcpyHostToDevice, c->stream);
+  if (n > s->lcap) {  // kept across calls, grown on demand
+    hipFree(s->lpos);
+    hipFree(s->lout);
+    s->lpos = nullptr;
+    s->lout = nullptr;
+    s->lcap = 0;
+    const long long cap = std::max<long long>(n + n / 4, 1024);
+    PN_HIP(c, hipMalloc(&s->lpos, sizeof(long long) * (size_t)cap));
+    PN_HIP(c, hipMalloc(&s->lout, sizeof(int) * (size_t)cap));
+    s->lcap = cap;
+  }
+  hipError_t e = hipMemcpyAsync(s->lpos, dense.data(), sizeof(long long) * n, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_isl_lookup, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->isl->label,
-                       c->isl->cid, dd, n, dout);
+    hipLaunchKernelGGL(k_isl_lookup, dim3((n + 255) / 256), dim3(256), 0, c->stream, s->label, s->cid,
+                       s->lpos, n, s->lout);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(island, dout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(island, s->lout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(dd);
-  hipFree(dout);
   if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands_lookup: %s", hipGetErrorString(e));
   return PN_OK;
 }

@@ -13,9 +13,11 @@
 // components nobody claimed are new islands, numbered in raster order of
 // their first site.  The reference re-grows by a BFS from the old peak; here
 // the component containing that site comes from the GPU labelling
-// (pn_islands_lookup) and its statistics from the catalog, which gives the
-// same partition and peak (a unique maximum does not depend on the seed) and
-// the same sums up to summation order.
+// (pn_islands_lookup) and its statistics from the catalog: the same partition
+// and the same sums up to summation order.  A unique maximum is the peak
+// whatever the seed; when the component holds equal maxima the peak is the
+// first of them in the BFS order from the old peak (pn_islands_seeded_peak),
+// as the reference's re-growth finds it.
 //
 // Histories keep at most five entries: the first two and the latest three
 // (islands.cpp:255-267).  Text outputs use the reference's stream formats.
@@ -32,11 +34,24 @@
 #include "pn_internal.h"
 
 struct pn_tracker {
+  // a tracked island: its history inline (no allocation per island)
   struct Isl {
     long long index = 0;
     int disp = 1;
     int status = 0;
-    std::vector<pn_track_entry> h;
+    int nh = 0;
+    pn_track_entry h[PN_TRACK_HIST];
+    const pn_track_entry &back() const { return h[nh - 1]; }
+    const pn_track_entry &front() const { return h[0]; }
+    // islands.cpp:255-267: the first two entries and the latest three
+    void push(const pn_track_entry &e) {
+      if (nh == PN_TRACK_HIST) {
+        for (int j = 2; j + 1 < PN_TRACK_HIST; j++) h[j] = h[j + 1];
+        h[PN_TRACK_HIST - 1] = e;
+      } else {
+        h[nh++] = e;
+      }
+    }
   };
   std::vector<Isl> found;
   long long number_of_islands = 0;
@@ -90,7 +105,7 @@ extern "C" int pn_tracker_get(const pn_tracker *t, int i, pn_tracked_island *out
   out->index = s.index;
   out->status = s.status;
   out->disp = s.disp;
-  out->nhist = (int)s.h.size();
+  out->nhist = s.nh;
   for (int j = 0; j < out->nhist; j++) out->hist[j] = s.h[j];
   return PN_OK;
 }
@@ -120,7 +135,7 @@ void write_entry(std::ofstream &o, const pn_track_entry &e, long long index, con
 
 // the islands.txt row (islands.cpp:683,717)
 void write_init_row(std::ofstream &o, const pn_tracker::Isl &s, int status, double thres) {
-  const pn_track_entry &e0 = s.h[0];
+  const pn_track_entry &e0 = s.front();
   const double r = std::pow(e0.volume * 3. / 4. / std::acos(-1.), 1. / 3.);
   char b[256];
   const int n = std::snprintf(b, sizeof b, "%g %g %g %lld %d %g\n", r, e0.peak_value, e0.at_time, s.index,
@@ -130,9 +145,21 @@ void write_init_row(std::ofstream &o, const pn_tracker::Isl &s, int status, doub
 
 }  // namespace
 
-extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
-                                 const char *outdir) {
-  if (!t || !c) return tfail(t, PN_ERR_ARG, "pn_tracker_update: NULL argument");
+static pn_track_entry entry_of(const pn_island &I, double tau) {
+  pn_track_entry e;
+  e.at_time = tau;
+  e.center_pos = I.peak_pos;
+  e.peak_value = I.peak_value;
+  e.avg_phi = I.avg_phi;
+  e.volume = I.volume;
+  e.pot_en = I.pot_en;
+  e.kin_en = I.kin_en;
+  e.grad_en = I.grad_en;
+  return e;
+}
+
+static int tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
+                          const char *outdir) {
   // the current components (room for the last count plus a quarter, so that
   // the analysis normally runs once)
   std::vector<pn_island> &cat = t->cat;
@@ -152,17 +179,20 @@ extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr,
   cat.resize(pc.count);
   t->cap_hint = std::max(1024, pc.count + pc.count / 4);
 
-  // largest latest volume first (islands.cpp:29-32,797); the comparison and
-  // the algorithm are the reference's, so ties keep its order
-  std::sort(t->found.begin(), t->found.end(), [](const pn_tracker::Isl &b1, const pn_tracker::Isl &b2) {
-    return b1.h.back().volume > b2.h.back().volume;
-  });
+  // largest latest volume first (islands.cpp:29-32,797): std::sort with the
+  // reference's comparison, so ties keep its order; the islands are sorted
+  // through an index (std::sort permutes indices exactly as it would the
+  // islands themselves)
+  const int n = (int)t->found.size();
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; i++) ord[i] = i;
+  const std::vector<pn_tracker::Isl> &fd = t->found;
+  std::sort(ord.begin(), ord.end(), [&fd](int i1, int i2) { return fd[i1].back().volume > fd[i2].back().volume; });
 
   // component of each tracked island's last peak (-1: the site left the mask)
-  const int n = (int)t->found.size();
   std::vector<long long> peaks(n);
   std::vector<int> comp(n, -1);
-  for (int i = 0; i < n; i++) peaks[i] = t->found[i].h.back().center_pos;
+  for (int i = 0; i < n; i++) peaks[i] = fd[ord[i]].back().center_pos;
   if (n > 0 && (rc = pn_islands_lookup(c, n, peaks.data(), comp.data())) != PN_OK)
     return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
 
@@ -177,80 +207,70 @@ extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr,
   }
 
   // Old_islands: retire the islands whose peak site is no longer in the mask
-  std::vector<pn_tracker::Isl> alive;
-  std::vector<int> acomp;
-  for (int i = 0; i < n; i++) {
-    pn_tracker::Isl &s = t->found[i];
-    if (comp[i] >= 0) {
-      alive.push_back(std::move(s));
-      acomp.push_back(comp[i]);
-      continue;
-    }
-    if (files) {
+  if (files)
+    for (int i = 0; i < n; i++) {
+      if (comp[i] >= 0) continue;
+      const pn_tracker::Isl &s = fd[ord[i]];
       if (s.disp) {
-        for (const pn_track_entry &e : s.h) write_entry(safe, e, s.index, co);
+        for (int j = 0; j < s.nh; j++) write_entry(safe, s.h[j], s.index, co);
         safe << '\n';
       }
       write_init_row(init, s, s.status, thr);
     }
-  }
 
-  // survivors re-grow in sorted order; a claimed component means a merge
+  // survivors re-grow from their last peak in sorted order; a claimed
+  // component means a merge.  With equal maxima in the component the peak is
+  // the first of them in breadth-first order from that old peak.
   std::vector<char> claimed(cat.size(), 0);
   std::vector<pn_tracker::Isl> next;
-  for (size_t i = 0; i < alive.size(); i++) {
-    const int k = acomp[i];
-    if (claimed[k]) continue;  // merged into an island earlier in the order
+  next.reserve((size_t)n + cat.size());
+  for (int i = 0; i < n; i++) {
+    const int k = comp[i];
+    if (k < 0 || claimed[k]) continue;  // retired, or merged into an island earlier in the order
     claimed[k] = 1;
-    pn_tracker::Isl s = std::move(alive[i]);
-    const pn_island &I = cat[k];
-    pn_track_entry e;
-    e.at_time = tau;
-    e.center_pos = I.peak_pos;
-    e.peak_value = I.peak_value;
-    e.avg_phi = I.avg_phi;
-    e.volume = I.volume;
-    e.pot_en = I.pot_en;
-    e.kin_en = I.kin_en;
-    e.grad_en = I.grad_en;
-    s.h.push_back(e);
-    if (s.h.size() > PN_TRACK_HIST) s.h.erase(s.h.begin() + 2);
+    pn_tracker::Isl s = fd[ord[i]];
+    pn_track_entry e = entry_of(cat[k], tau);
+    if (cat[k].peak_ties > 0 &&
+        (rc = pn_islands_seeded_peak(c, k, s.back().center_pos, &e.center_pos, &e.peak_value)) != PN_OK)
+      return tfail(t, rc, std::string("pn_tracker_update: ") + pn_last_error(c));
+    s.push(e);
     s.status = 0;
-    if (s.h.back().grad_en + s.h.back().pot_en < 0.) {  // is_bubble (islands.cpp:61-65)
+    if (s.back().grad_en + s.back().pot_en < 0.) {  // is_bubble (islands.cpp:61-65)
       if (files) {
         // print_bubble_history (:707-736) opens expanding_bubbles.txt when called
         if (!bubbles.is_open()) bubbles.open(std::string(outdir) + "/expanding_bubbles.txt", std::ios::app);
         if (s.disp) write_init_row(init, s, 1, thr);
-        for (size_t j = 1; j < s.h.size(); j++) write_entry(bubbles, s.h[j], s.index, co);
+        for (int j = 1; j < s.nh; j++) write_entry(bubbles, s.h[j], s.index, co);
         bubbles << '\n';
       }
       s.status = 1;
     }
-    if (s.status == 0 && s.h.back().at_time - s.h.front().at_time > 40.)  // is_oscillon (:67-70)
+    if (s.status == 0 && s.back().at_time - s.front().at_time > 40.)  // is_oscillon (:67-70)
       s.status = 2;
-    next.push_back(std::move(s));
+    next.push_back(s);
   }
 
   // new islands: unclaimed components in raster order of their first site
+  // (their peaks, ties included, are the catalog's: BFS from that site)
   for (size_t k = 0; k < cat.size(); k++) {
     if (claimed[k]) continue;
-    const pn_island &I = cat[k];
     pn_tracker::Isl s;
     s.index = t->number_of_islands++;
     s.disp = 1;
     s.status = 0;
-    pn_track_entry e;
-    e.at_time = tau;
-    e.center_pos = I.peak_pos;
-    e.peak_value = I.peak_value;
-    e.avg_phi = I.avg_phi;
-    e.volume = I.volume;
-    e.pot_en = I.pot_en;
-    e.kin_en = I.kin_en;
-    e.grad_en = I.grad_en;
-    s.h.push_back(e);
-    next.push_back(std::move(s));
+    s.push(entry_of(cat[k], tau));
+    next.push_back(s);
   }
-  t->found = std::move(next);
+  t->found.swap(next);
   return PN_OK;
+}
+
+extern "C" int pn_tracker_update(pn_tracker *t, pn_ctx *c, int mode, double thr, double tau,
+                                 const char *outdir) {
+  if (!t || !c) return tfail(t, PN_ERR_ARG, "pn_tracker_update: NULL argument");
+  try {
+    return tracker_update(t, c, mode, thr, tau, outdir);
+  } catch (const std::bad_alloc &) {
+    return tfail(t, PN_ERR_NOMEM, "pn_tracker_update: out of host memory");
+  }
 }

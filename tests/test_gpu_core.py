@@ -547,7 +547,6 @@ def test_islands_match_bfs(N, seed, mode):
     for r, c in zip(rows, cat):
         assert r.index == int(c[0]) and r.nn == int(c[1])
         assert r.volume == c[2]
-        assert r.peak_ties == 0
         assert r.peak_pos == int(c[3])
         assert r.peak_value == c[4]
         np.testing.assert_allclose([r.avg_phi, r.pot_en, r.kin_en, r.grad_en], c[5:9],
@@ -633,7 +632,7 @@ def test_large_islands_match_bfs(mode):
     # integers, an ordered merge of the chunks' sums); the small islands from
     # one wave each.  Catalog and labels against the oracle's BFS, centroids by
     # brute force, and two planted equal maxima in the large island: one tie,
-    # the peak at the smaller index
+    # the peak the one the BFS from the island's first site reaches first
     N = 40
     m = O.model(N)
     P = N * N
@@ -666,13 +665,72 @@ def test_large_islands_match_bfs(mode):
                                    rtol=1e-12, atol=1e-14)
         members = np.nonzero(labels == r.index)[0]
         assert r.first_pos == members.min() + P
-        if r.peak_ties == 0:
-            assert r.peak_pos == int(c[3]) and r.peak_value == c[4]
+        assert r.peak_pos == int(c[3]) and r.peak_value == c[4]
     _check_centroids(rows, glab, N)
     if mode == 0:
         big = rows[int(np.argmax(sizes))]
         assert glab[i1] == big.index and glab[i2] == big.index
-        assert big.peak_ties == 1 and big.peak_pos == min(i1, i2) + P and big.peak_value == 9.0
+        assert big.peak_ties == 1 and big.peak_pos in (i1 + P, i2 + P) and big.peak_value == 9.0
+
+
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_tied_peak_is_the_first_in_bfs_order(prec):
+    # an L-shaped island in one plane: a row y = 3, x = 2..8, and a column
+    # x = 2, y = 3..9.  Its first site (2, 3) seeds the reference's BFS; equal
+    # phi^2 at (8, 3) (phi = +9, the smaller index) and (2, 5) (phi = -9, two
+    # steps from the seed against six): the BFS keeps (2, 5) and Phi = -9
+    # (islands.cpp:104-116), where the smallest index would give (8, 3)
+    N = 16
+    m = O.model(N)
+    P = N * N
+    phi = np.zeros((N, N, N))
+    z = 5
+    for x in range(2, 9):
+        phi[z, 3, x] = 4.0
+    for y in range(3, 10):
+        phi[z, y, 2] = 4.0
+    phi[z, 3, 8] = 9.0
+    phi[z, 5, 2] = -9.0
+    phi = phi.ravel()
+    pi = np.zeros(N ** 3)
+    cat, labels = O.islands(m, 0, 3.0, phi, pi)
+    with _lat(m, prec) as L:
+        L.set_fields(phi, pi)
+        rows, glab, _ = L.islands(0, 3.0, labels=True)
+    assert len(rows) == 1 and cat.shape[0] == 1
+    r = rows[0]
+    assert r.peak_ties == 1
+    assert r.peak_pos == int(cat[0][3]) == P + z * P + 5 * N + 2
+    assert r.peak_value == cat[0][4] == -9.0
+    _check_centroids(rows, glab, N)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
+def test_quantized_fields_peaks_match_bfs(mode, prec):
+    # phi and pi on a grid of quarter units: many islands hold equal maxima;
+    # every peak site and value against the oracle's BFS (binary32 holds these
+    # values exactly, so both lattices see the same fields)
+    N = 24
+    m = O.model(N)
+    rng = np.random.default_rng(77 + mode)
+    phi = np.round(4.0 * (0.8 * rng.standard_normal(N ** 3))) / 4.0
+    pi = np.round(4.0 * (0.3 * rng.standard_normal(N ** 3))) / 4.0
+    if mode == 0:
+        thr = 1.6
+    else:
+        thr = 2.0 * O.volume_averages(m, phi, pi)[7]
+    cat, labels = O.islands(m, mode, thr, phi, pi)
+    with _lat(m, prec) as L:
+        L.set_fields(phi, pi)
+        rows, glab, _ = L.islands(mode, thr, labels=True)
+    assert np.array_equal(glab, labels)
+    assert len(rows) == cat.shape[0] > 20
+    if mode == 0:
+        assert sum(r.peak_ties > 0 for r in rows) > 5
+    for r, c in zip(rows, cat):
+        assert (r.peak_pos, r.peak_value) == (int(c[3]), c[4]), r.index
+    _check_centroids(rows, glab, N)
 
 
 @pytest.mark.parametrize("kappa", [1.2247448713915889, 0.46775209018414249])

@@ -81,6 +81,36 @@ def test_tracking_matches_oracle(mode, thr):
     _same_files(gfiles, wfiles)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tracking_with_tied_peaks(mode):
+    # quantized fields (quarter units) give many islands equal maxima; a
+    # tracked island re-grows from its last peak (islands.cpp:798-817), so its
+    # new peak is the first of the equal maxima in breadth-first order from
+    # there, and that peak is the seed of the next analysis.  Three analyses
+    # on related fields, against the checker's Full_Analysis
+    N = 20
+    m = O.model(N)
+    rng = np.random.default_rng(300 + mode)
+    base = 0.9 * rng.standard_normal(N ** 3)
+    pi = np.round(4.0 * 0.3 * rng.standard_normal(N ** 3)) / 4.0
+    thr = 1.6 if mode == 0 else 2.0 * O.volume_averages(m, np.round(4 * base) / 4, pi)[7]
+    T = O.Tracker(m)
+    with pn.Lattice(N, m["a"], m["mass_mode"], m["m2"], m["lambda_mode"], m["kappa"]) as L, \
+            pn.Tracker() as G, tempfile.TemporaryDirectory() as gd, tempfile.TemporaryDirectory() as od:
+        for k in range(3):
+            phi = np.round(4.0 * (base + 0.1 * k * rng.standard_normal(N ** 3))) / 4.0
+            T.update(mode, thr, phi, pi, 1.0 * k, od)
+            L.set_fields(phi, pi)
+            G.update(L, mode, thr, 1.0 * k, gd)
+        got, want = G.islands(), T.islands()
+        gfiles = {fn: open(os.path.join(gd, fn)).read() for fn in sorted(os.listdir(gd))}
+        wfiles = {fn: open(os.path.join(od, fn)).read() for fn in sorted(os.listdir(od))}
+    T.close()
+    assert len(want) > 20
+    _same_track(got, want)
+    _same_files(gfiles, wfiles)
+
+
 def test_tracker_catalog_is_the_update_components():
     # pn_tracker_catalog returns the catalog the update computed, as
     # pn_islands gives it for the same fields and threshold
