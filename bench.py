@@ -369,7 +369,8 @@ def analysis_only(pn, N, reps=10):
         calls = {"volume_analysis": lambda: L.volume_analysis(),
                  "volume_averages_synced": lambda: L.volume_averages_synced(0.025),
                  "spectrum": lambda: L.spectrum(),
-                 "islands": lambda: L.islands(0, thr, as_array=True)}
+                 "islands": lambda: L.islands(0, thr, as_array=True, view=True),
+                 "islands_with_labels": lambda: L.islands(0, thr, labels=True)}
         for name, f in calls.items():
             f()
             t0 = time.perf_counter()
@@ -377,6 +378,17 @@ def analysis_only(pn, N, reps=10):
                 f()
             out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
             log("analysis %d^3 %s: %.3f ms" % (N, name, out[name + "_ms"]))
+        # Full_Analysis repeated on the same field: every island is tracked
+        # from the second call on (re-grown from its peak, none retired)
+        with pn.Tracker() as T:
+            T.update(L, 0, thr, 0.0)
+            t0 = time.perf_counter()
+            for k in range(reps):
+                T.update(L, 0, thr, 1.0 + k)
+            out["tracker_update_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+            out["tracked_islands"] = len(T.islands())
+        log("analysis %d^3 tracker update: %.3f ms (%d islands)" % (N, out["tracker_update_ms"],
+                                                                  out["tracked_islands"]))
     return out
 
 
@@ -598,6 +610,8 @@ def main():
                     help="S:Z fixes the FP64 multi-step kernel (S = 2 or 3 steps per pass, z block Z) "
                          "instead of the library's timed choice (profiling runs)")
     ap.add_argument("--analysis", type=int, default=0, help="only time the analysis calls at N^3")
+    ap.add_argument("--cfg1-only", action="store_true",
+                    help="only the reference's loop with outputs at 256^3 (--cfg1-steps steps)")
     ap.add_argument("--fp32-only", action="store_true",
                     help="only the FP32 form at --size for --steps steps (for profiling)")
     ap.add_argument("--single-step", action="store_true", help="time the one-step FP64 kernel only")
@@ -628,6 +642,9 @@ def main():
     kern = tuple(int(v) for v in args.kernel.split(":")) if args.kernel else None
     if args.analysis:
         print(json.dumps(analysis_only(pn, args.analysis)), flush=True)
+        return
+    if args.cfg1_only:
+        print(json.dumps(cfg1_leg(pn, 256, args.cfg1_steps)), flush=True)
         return
     m = model(lam=1.2, g=0.25, phi0=1.2)
     if args.fp32_only or args.single_step:

@@ -20,6 +20,8 @@
 #ifndef PHINUM_GPU_H
 #define PHINUM_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -269,6 +271,13 @@ int pn_fft_modes(pn_ctx *c, int field, double *out);
  *         peak = max Delta.
  * Connectivity: 6 neighbours, periodic in x and y, open in z (islands.cpp:200-236). */
 int pn_islands(pn_ctx *c, int mode, double thr, pn_catalog *out);
+/* Page-locked host memory for output arrays that are filled on every call
+ * (island catalogs: about 33 MB for 233k islands at 256^3), so that their copy
+ * from the device runs at the link's rate instead of through pageable
+ * staging.  No reference counterpart (the reference's catalog never leaves
+ * host memory).  Free with pn_host_free. */
+int pn_host_alloc(size_t bytes, void **out);
+int pn_host_free(void *p);
 /* Island number (row of the last pn_islands catalog) of each site pos[i]
  * (reference pos() index of an interior site), or -1 outside the mask.
  * Replaces the b[] array lookups of Full_Analysis (islands.cpp:798-817). */

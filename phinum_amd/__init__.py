@@ -146,6 +146,8 @@ def lib():
         "pn_fft_modes": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pn_islands": (C.c_int, [C.c_void_p, C.c_int, C.c_double, P(PnCatalog)]),
         "pn_islands_lookup": (C.c_int, [C.c_void_p, C.c_int, P(C.c_longlong), P(C.c_int)]),
+        "pn_host_alloc": (C.c_int, [C.c_size_t, P(C.c_void_p)]),
+        "pn_host_free": (C.c_int, [C.c_void_p]),
         "pn_tracker_create": (C.c_int, [P(C.c_void_p)]),
         "pn_tracker_destroy": (C.c_int, [C.c_void_p]),
         "pn_tracker_last_error": (C.c_char_p, [C.c_void_p]),
@@ -233,10 +235,26 @@ class Lattice:
         return rc
 
     def close(self):
+        if getattr(self, "_hrows", None):
+            self.L.pn_host_free(self._hrows)
+            self._hrows = None
+            self._hrows_cap = 0
         if getattr(self, "h", None):
             if getattr(self, "_owned", True):
                 self.L.pn_destroy(self.h)
             self.h = None
+
+    def _host_rows(self, capacity):
+        """A page-locked row array (pn_host_alloc) kept across islands calls."""
+        if getattr(self, "_hrows_cap", 0) < capacity:
+            if getattr(self, "_hrows", None):
+                self.L.pn_host_free(self._hrows)
+                self._hrows = None
+                self._hrows_cap = 0
+            p = C.c_void_p()
+            self._chk(self.L.pn_host_alloc(C.sizeof(PnIsland) * capacity, C.byref(p)))
+            self._hrows, self._hrows_cap = p, capacity
+        return C.cast(self._hrows, C.POINTER(PnIsland))
 
     def two_step(self, on=None):
         """Gets or sets PN_OPT_TWO_STEP (pairs of steps per pass over HBM)."""
@@ -348,18 +366,20 @@ class Lattice:
         self._chk(self.L.pn_fft_modes(self.h, int(field), _dptr(out)))
         return out.view(np.complex128).reshape(self.N, self.N, H)
 
-    def islands(self, mode, thr, capacity=None, labels=False, grow=True, as_array=False):
+    def islands(self, mode, thr, capacity=None, labels=False, grow=True, as_array=False, view=False):
         """Island catalog (rows of PnIsland, or with as_array one numpy
-        structured array over the same memory), the label array if asked for,
-        and the threshold used.  capacity defaults to the last catalog's size
-        plus a quarter (at least 65536).  With grow, a catalog larger than
-        `capacity` is fetched again with room for all of it (the library
-        reports the count with PN_ERR_CAPACITY); without, that error is
-        raised."""
+        structured array), the label array if asked for, and the threshold
+        used.  The rows arrive in a page-locked array kept by the lattice;
+        they are returned as copies, or with as_array and view as a numpy
+        view of that array, valid until the next islands call.  capacity
+        defaults to the last catalog's size plus a quarter (at least 65536).
+        With grow, a catalog larger than `capacity` is fetched again with
+        room for all of it (the library reports the count with
+        PN_ERR_CAPACITY); without, that error is raised."""
         if capacity is None:
             capacity = getattr(self, "_isl_cap", 65536)
         while True:
-            arr = (PnIsland * max(capacity, 1))()
+            arr = self._host_rows(max(capacity, 1))
             lab = np.empty(self.N ** 3, dtype=np.int32) if labels else None
             cat = PnCatalog(capacity, 0, 0.0, arr,
                             lab.ctypes.data_as(C.POINTER(C.c_int)) if labels else None)
@@ -370,9 +390,11 @@ class Lattice:
             self._chk(rc)
             self._isl_cap = max(65536, cat.count + cat.count // 4)
             if as_array:
-                rows = np.ctypeslib.as_array(arr)[:cat.count]
+                rows = np.ctypeslib.as_array(arr, shape=(max(cat.count, 1),))[:cat.count]
+                if not view:
+                    rows = rows.copy()
             else:
-                rows = [arr[i] for i in range(cat.count)]
+                rows = [PnIsland.from_buffer_copy(arr[i]) for i in range(cat.count)]
             return rows, lab, cat.threshold
 
     def initial_conditions(self, ic):

@@ -276,7 +276,7 @@ int check_params(const pn_adj_params *p, int nsteps) {
   if (!p || p->N < 4 || p->N > 1024 || !(p->a > 0) || nsteps < 0 || !(p->width > 0))
     return afail(PN_ERR_ARG, "adjoint: bad parameters");
   if (p->checkpoint_every < 1) return afail(PN_ERR_ARG, "adjoint: checkpoint_every must be >= 1");
-  return PN_OK;
+  return pn_runtime_check();
 }
 
 AdjCoef coef(const pn_adj_params *p) {

@@ -499,6 +499,7 @@ extern "C" int pn_batch_create(int R, const pn_realization *reals, int precision
                                pn_batch **out) {
   if (!out || !reals || R < 1) return bfail(nullptr, PN_ERR_ARG, "pn_batch_create: bad argument");
   *out = nullptr;
+  if (const int rc = pn_runtime_check()) return rc;
   if (precision != PN_FP64 && precision != PN_FP32)
     return bfail(nullptr, PN_ERR_ARG, "pn_batch_create: unknown precision");
   pn_batch *b = new (std::nothrow) pn_batch();

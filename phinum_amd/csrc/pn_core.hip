@@ -15,9 +15,13 @@
 // with the same result bits -- and is compiled with -ffp-contract=off, so each
 // site's update is bit-identical to a strict build of the reference.
 #include <hip/hip_runtime.h>
+#include <hip/hip_version.h>
+#include <rocfft/rocfft-version.h>
+#include <rocfft/rocfft.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -516,6 +520,64 @@ static constexpr int kVolMaxTiles = 16384;
 // devices a process may use with the per-device caches below
 static constexpr int kScratchMaxDevices = 64;
 
+// A process that loaded another HIP runtime or rocFFT before this library (a
+// framework that bundles its own ROCm libraries under the same sonames) binds
+// the library's calls to those: rocFFT plans described through this build's
+// headers are then executed by another version's implementation, and code
+// objects built by this compiler are loaded by another runtime.  Refused
+// before any device work.
+int pn_runtime_check() {
+  static std::once_flag once;
+  static int rc = PN_OK;
+  static char msg[256];
+  std::call_once(once, [] {
+    int v = 0;
+    if (hipRuntimeGetVersion(&v) != hipSuccess) {
+      rc = PN_ERR_HIP;
+      std::snprintf(msg, sizeof msg, "hipRuntimeGetVersion failed");
+      return;
+    }
+    if (v / 100000 != HIP_VERSION / 100000) {
+      rc = PN_ERR_STATE;
+      std::snprintf(msg, sizeof msg,
+                    "the process loaded HIP runtime %d.%d, this library is built for HIP %d.%d: load it "
+                    "before any other ROCm library, in a process of its own",
+                    v / 10000000, (v / 100000) % 100, HIP_VERSION_MAJOR, HIP_VERSION_MINOR);
+      return;
+    }
+    char got[128] = {0}, want[64];
+    std::snprintf(want, sizeof want, "%d.%d.%d.", rocfft_version_major, rocfft_version_minor,
+                  rocfft_version_patch);
+    if (rocfft_get_version_string(got, sizeof got - 1) != rocfft_status_success ||
+        std::strncmp(got, want, std::strlen(want)) != 0) {
+      rc = PN_ERR_STATE;
+      std::snprintf(msg, sizeof msg,
+                    "the process loaded rocFFT %s, this library is built for rocFFT %s: load it before "
+                    "any other ROCm library, in a process of its own",
+                    got, want);
+    }
+  });
+  if (rc != PN_OK) return pn_fail(nullptr, rc, "%s", msg);
+  return PN_OK;
+}
+
+extern "C" int pn_host_alloc(size_t bytes, void **out) {
+  if (!out) return pn_fail(nullptr, PN_ERR_ARG, "pn_host_alloc: out is NULL");
+  *out = nullptr;
+  if (bytes == 0) return pn_fail(nullptr, PN_ERR_ARG, "pn_host_alloc: zero bytes");
+  if (const int rc = pn_runtime_check()) return rc;
+  if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) {
+    *out = nullptr;
+    return pn_fail(nullptr, PN_ERR_NOMEM, "pn_host_alloc: %zu bytes of page-locked memory", bytes);
+  }
+  return PN_OK;
+}
+
+extern "C" int pn_host_free(void *p) {
+  if (p && hipHostFree(p) != hipSuccess) return pn_fail(nullptr, PN_ERR_HIP, "pn_host_free failed");
+  return PN_OK;
+}
+
 extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
   if (!out) return pn_fail(nullptr, PN_ERR_ARG, "pn_create: out is NULL");
   *out = nullptr;
@@ -528,6 +590,7 @@ extern "C" int pn_create(const pn_params *p, pn_ctx **out) {
     return pn_fail(nullptr, PN_ERR_ARG, "pn_create: unknown precision %d", p->precision);
   if (!(p->kappa != 0) || !std::isfinite(p->kappa))
     return pn_fail(nullptr, PN_ERR_ARG, "pn_create: Kappa must be finite and nonzero");
+  if (const int rc = pn_runtime_check()) return rc;
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0)

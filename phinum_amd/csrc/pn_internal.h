@@ -73,6 +73,11 @@ int pn_make_view(const pn_params *p, void *phi0, void *phi1, void *pi, int cur, 
 // dA, dE hold N*N*(N/2+1) double2 each; runs on c->stream and waits for it
 int pn_icgen_modes(pn_ctx *c, const pn_ic_params *ic, void *dA, void *dE);
 
+// The HIP runtime and rocFFT loaded into the process must be the versions the
+// library was compiled against (PN_OK, or PN_ERR_STATE with the versions in
+// the message); checked once per process, before any device work
+int pn_runtime_check();
+
 // error helpers ------------------------------------------------------------
 int pn_fail(pn_ctx *c, int code, const char *fmt, ...);
 void pn_set_thread_error(const char *msg);

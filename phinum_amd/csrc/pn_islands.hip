@@ -66,9 +66,10 @@ struct pn_island_state {
   pn_island *rows = nullptr;
   long long rcap = 0;
   int *dlab = nullptr;
-  // pn_islands_lookup's positions and answers (grown on demand)
-  long long *lpos = nullptr;
-  int *lout = nullptr;
+  // pn_islands_lookup's positions and answers on the device and their
+  // page-locked host staging (grown on demand)
+  long long *lpos = nullptr, *hpos = nullptr;
+  int *lout = nullptr, *hout = nullptr;
   long long lcap = 0;
   // one island's keys and peak values for the breadth-first order of tied peaks
   double *kv = nullptr;
@@ -107,6 +108,8 @@ void pn_free_islands(pn_ctx *c) {
   hipFree(s->dlab);
   hipFree(s->lpos);
   hipFree(s->lout);
+  hipHostFree(s->hpos);
+  hipHostFree(s->hout);
   hipFree(s->kv);
   delete s;
   c->isl = nullptr;
@@ -1081,37 +1084,35 @@ extern "C" int pn_islands_lookup(pn_ctx *c, int n, const long long *pos, int *is
   if (n == 0) return PN_OK;
   pn_island_state *s = c->isl;
   const long long P = (long long)c->p.N * c->p.N;
-  std::vector<long long> dense;
-  try {
-    dense.resize(n);
-  } catch (const std::bad_alloc &) {
-    return pn_fail(c, PN_ERR_NOMEM, "pn_islands_lookup: no host memory for %d positions", n);
-  }
-  for (int i = 0; i < n; i++) {
-    dense[i] = pos[i] - P;  // reference pos() (one z halo plane below) -> dense
-    if (dense[i] < 0 || dense[i] >= c->n3)
+  for (int i = 0; i < n; i++)
+    if (pos[i] - P < 0 || pos[i] - P >= c->n3)
       return pn_fail(c, PN_ERR_ARG, "pn_islands_lookup: pos %lld is not an interior site", pos[i]);
-  }
   PN_HIP(c, hipSetDevice(c->p.device));
   if (n > s->lcap) {  // kept across calls, grown on demand
     hipFree(s->lpos);
     hipFree(s->lout);
-    s->lpos = nullptr;
-    s->lout = nullptr;
+    hipHostFree(s->hpos);
+    hipHostFree(s->hout);
+    s->lpos = s->hpos = nullptr;
+    s->lout = s->hout = nullptr;
     s->lcap = 0;
     const long long cap = std::max<long long>(n + n / 4, 1024);
     PN_HIP(c, hipMalloc(&s->lpos, sizeof(long long) * (size_t)cap));
     PN_HIP(c, hipMalloc(&s->lout, sizeof(int) * (size_t)cap));
+    PN_HIP(c, hipHostMalloc(&s->hpos, sizeof(long long) * (size_t)cap, hipHostMallocDefault));
+    PN_HIP(c, hipHostMalloc(&s->hout, sizeof(int) * (size_t)cap, hipHostMallocDefault));
     s->lcap = cap;
   }
-  hipError_t e = hipMemcpyAsync(s->lpos, dense.data(), sizeof(long long) * n, hipMemcpyHostToDevice, c->stream);
+  for (int i = 0; i < n; i++) s->hpos[i] = pos[i] - P;  // reference pos() (one z halo plane below) -> dense
+  hipError_t e = hipMemcpyAsync(s->lpos, s->hpos, sizeof(long long) * n, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_isl_lookup, dim3((n + 255) / 256), dim3(256), 0, c->stream, s->label, s->cid,
                        s->lpos, n, s->lout);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(island, s->lout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->hout, s->lout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return pn_fail(c, PN_ERR_HIP, "pn_islands_lookup: %s", hipGetErrorString(e));
+  std::copy(s->hout, s->hout + n, island);
   return PN_OK;
 }

@@ -1,6 +1,8 @@
 """GPU parity of the fused leapfrog step, reductions, spectrum and islands
 against the oracle (bitwise where the reference's arithmetic is reproduced,
 stated tolerances where the summation order necessarily differs)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -671,6 +673,41 @@ def test_large_islands_match_bfs(mode):
         big = rows[int(np.argmax(sizes))]
         assert glab[i1] == big.index and glab[i2] == big.index
         assert big.peak_ties == 1 and big.peak_pos in (i1 + P, i2 + P) and big.peak_value == 9.0
+
+
+def test_catalog_into_pageable_and_page_locked_arrays():
+    # the same rows, byte for byte, into a caller's pageable array and into
+    # page-locked memory from pn_host_alloc, through the C ABI; then a
+    # catalog larger than the array: the count, PN_ERR_CAPACITY and the
+    # first rows only
+    N = 32
+    m = O.model(N)
+    phi, pi = _fields(N, 57, phi0=0.0, amp=1.0)
+    sz = C.sizeof(pn.PnIsland)
+    n = N ** 3
+    with _lat(m) as L:
+        L.set_fields(phi, pi)
+        buf = np.zeros(n * sz, np.uint8)
+        cat = pn.PnCatalog(n, 0, 0.0, C.cast(buf.ctypes.data, C.POINTER(pn.PnIsland)), None)
+        assert L.L.pn_islands(L.h, 0, 1.5, C.byref(cat)) == pn.PN_OK
+        assert cat.count > 100
+        p = C.c_void_p()
+        assert L.L.pn_host_alloc(n * sz, C.byref(p)) == pn.PN_OK
+        try:
+            catp = pn.PnCatalog(n, 0, 0.0, C.cast(p, C.POINTER(pn.PnIsland)), None)
+            assert L.L.pn_islands(L.h, 0, 1.5, C.byref(catp)) == pn.PN_OK
+            assert catp.count == cat.count
+            assert C.string_at(p, cat.count * sz) == buf[:cat.count * sz].tobytes()
+            few = pn.PnCatalog(7, 0, 0.0, C.cast(p, C.POINTER(pn.PnIsland)), None)
+            C.memset(p, 0, n * sz)
+            assert L.L.pn_islands(L.h, 0, 1.5, C.byref(few)) == pn.PN_ERR_CAPACITY
+            assert few.count == cat.count
+            assert C.string_at(p, 7 * sz) == buf[:7 * sz].tobytes()
+            assert C.string_at(p.value + 7 * sz, sz) == bytes(sz)
+        finally:
+            assert L.L.pn_host_free(p) == pn.PN_OK
+        rows, _, _ = L.islands(0, 1.5, as_array=True)
+        assert rows.tobytes() == buf[:cat.count * sz].tobytes()
 
 
 @pytest.mark.parametrize("prec", [pn.PN_FP64, pn.PN_FP32])
