@@ -368,9 +368,10 @@ def analysis_only(pn, N, reps=10):
         thr = 2.57583 * np.sqrt(L.volume_averages().phi2)
         calls = {"volume_analysis": lambda: L.volume_analysis(),
                  "volume_averages_synced": lambda: L.volume_averages_synced(0.025),
-                 "spectrum": lambda: L.spectrum(),
-                 "islands": lambda: L.islands(0, thr, as_array=True, view=True),
-                 "islands_with_labels": lambda: L.islands(0, thr, labels=True)}
+                 "spectrum": lambda: L.spectrum()}
+        if N <= 512:  # catalogs of millions of islands above that
+            calls["islands"] = lambda: L.islands(0, thr, as_array=True, view=True)
+            calls["islands_with_labels"] = lambda: L.islands(0, thr, labels=True)
         for name, f in calls.items():
             f()
             t0 = time.perf_counter()
@@ -378,6 +379,8 @@ def analysis_only(pn, N, reps=10):
                 f()
             out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
             log("analysis %d^3 %s: %.3f ms" % (N, name, out[name + "_ms"]))
+        if N > 512:
+            return out
         # Full_Analysis repeated on the same field: every island is tracked
         # from the second call on (re-grown from its peak, none retired)
         with pn.Tracker() as T:
