@@ -478,6 +478,16 @@ def adjoint_leg(pn, N=128, steps=4000, K=100, lam=1.0, g=0.2, check_steps=1000, 
     out.update({"F": F, "dF_dlambda": -gc[1], "dF_dg": gc[2], "forward_ms": res.forward_ms,
                 "backward_ms": res.backward_ms, "backward_over_forward": res.backward_ms / res.forward_ms,
                 "peak_device_bytes": res.peak_bytes})
+    # every phi_n resident in HBM (checkpoint_every >= steps): no recompute;
+    # the same kernels in the same order, so the same bits
+    pr = pn.adj_params(N, c2=1.0, c4=-lam, c6=g, checkpoint_every=steps)
+    Fr, gphir, _, gcr, resr = pn.adjoint(pr, phi, pi, dtk, dtd)
+    out["resident"] = {"forward_ms": resr.forward_ms, "backward_ms": resr.backward_ms,
+                       "backward_over_forward": resr.backward_ms / resr.forward_ms,
+                       "peak_device_bytes": resr.peak_bytes,
+                       "same_bits_as_checkpointed": bool(Fr == F and np.array_equal(gphir, gphi)
+                                                         and np.array_equal(gcr, gc))}
+    del gphir
     conv = {}
     for e in (1e-6, 1e-7, 1e-8):
         conv["%g" % e] = (fwd(lam + e, g, phi, dtk, dtd) - fwd(lam - e, g, phi, dtk, dtd)) / (2 * e)
@@ -613,6 +623,7 @@ def main():
                     help="S:Z fixes the FP64 multi-step kernel (S = 2 or 3 steps per pass, z block Z) "
                          "instead of the library's timed choice (profiling runs)")
     ap.add_argument("--analysis", type=int, default=0, help="only time the analysis calls at N^3")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the 128^3 adjoint leg")
     ap.add_argument("--cfg1-only", action="store_true",
                     help="only the reference's loop with outputs at 256^3 (--cfg1-steps steps)")
     ap.add_argument("--fp32-only", action="store_true",
@@ -645,6 +656,9 @@ def main():
     kern = tuple(int(v) for v in args.kernel.split(":")) if args.kernel else None
     if args.analysis:
         print(json.dumps(analysis_only(pn, args.analysis)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(pn)), flush=True)
         return
     if args.cfg1_only:
         print(json.dumps(cfg1_leg(pn, 256, args.cfg1_steps)), flush=True)

@@ -415,7 +415,10 @@ int pn_forward_observable(const pn_adj_params *p, const double *phi0, const doub
                           const double *dtk, const double *dtd, double *F);
 /* Reverse entry point: F, dF/dphi0 and dF/dpi0 (N^3 each, either may be
  * NULL) and dF/dc in res, by checkpointed recompute (every
- * checkpoint_every steps) and a hand-written adjoint stencil. */
+ * checkpoint_every steps) and a hand-written adjoint stencil.  With
+ * checkpoint_every >= nsteps every phi_n stays in device memory from the
+ * forward pass (nsteps * N^3 doubles) and nothing is recomputed; the
+ * results are the same bits either way. */
 int pn_adjoint(const pn_adj_params *p, const double *phi0, const double *pi0, int nsteps,
                const double *dtk, const double *dtd, double *grad_phi0, double *grad_pi0,
                pn_adj_result *res);

@@ -15,8 +15,12 @@
 // Reverse pass, per step n from S-1 down to 0, with pb = pibar + dtd[n]*phibar:
 //   phibar <- phibar + dtk[n] * (Lap pb - f'(phi_n) pb),   pibar <- pb,
 //   cbar   -= dtk[n] * sum pb * (phi, q phi, q^2 phi)
+// pb is formed once per site: the reverse step for n also writes the next
+// step's pb = pibar + dtd[n-1]*phibar, so the stencil reads one array.
 // phi_n comes from checkpoints of (phi, pi) every K steps: each segment is
-// recomputed forward into a K-plane store, then walked backwards.
+// recomputed forward into a K-plane store, then walked backwards.  With
+// K >= S every phi_n stays in HBM from the forward pass (S planes, 67 GB for
+// 4000 steps at 128^3) and nothing is recomputed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,6 +39,9 @@ struct AdjCoef {
 
 constexpr int kT = 256;
 constexpr int kBlocks = 1024;
+// planes per block of the z-streaming step kernels (both directions); 2, 4
+// and 8 planes ran at the same rate at 128^3
+constexpr int kZB = 16;
 
 __device__ __forceinline__ long long nb_idx(long long i, int x, int y, int z, int N, int d, int s) {
   // neighbour of site i in direction d (0 x, 1 y, 2 z) with step s = +-1, periodic
@@ -54,13 +61,13 @@ __device__ __forceinline__ long long nb_idx(long long i, int x, int y, int z, in
   return i + (long long)(zz - z) * P;
 }
 
-// forward step, z-streaming columns as the main step kernel, polynomial force
+// forward step, z-streaming columns of kZB planes, polynomial force
 __global__ __launch_bounds__(256) void k_poly_step(const double *__restrict__ phi,
                                                    double *__restrict__ pi,
                                                    double *__restrict__ phin, AdjCoef k, double dtk,
                                                    double dtd) {
   const int N = k.N;
-  const int XT = (N + 63) / 64, YT = (N + 3) / 4, ZB = 16, ZT = (N + ZB - 1) / ZB;
+  const int XT = (N + 63) / 64, YT = (N + 3) / 4, ZB = kZB, ZT = (N + ZB - 1) / ZB;
   const int b = blockIdx.x;
   if (b >= XT * YT * ZT) return;
   const int xt = b % XT, yt = (b / XT) % YT, zt = b / (XT * YT);
@@ -92,7 +99,7 @@ __global__ __launch_bounds__(256) void k_poly_step(const double *__restrict__ ph
   }
 }
 
-static int poly_blocks(int N) { return ((N + 63) / 64) * ((N + 3) / 4) * ((N + 15) / 16); }
+static int poly_blocks(int N) { return ((N + 63) / 64) * ((N + 3) / 4) * ((N + kZB - 1) / kZB); }
 
 __device__ __forceinline__ double site_rho(const double *phi, const double *pi, long long i, int x,
                                            int y, int z, const AdjCoef &k) {
@@ -198,43 +205,75 @@ __global__ void k_obs_back(const double *__restrict__ phi, const double *__restr
   }
 }
 
-// one reverse step; the coupling partials go to the block's own slot
-__global__ __launch_bounds__(kT) void k_adj_step(const double *__restrict__ phin_,
-                                                 const double *__restrict__ bphi,
-                                                 const double *__restrict__ bpi,
-                                                 double *__restrict__ bphi_o,
-                                                 double *__restrict__ bpi_o, AdjCoef k, double dtk,
-                                                 double dtd, double *__restrict__ gacc) {
+// One reverse step as a z-streaming stencil on pb, in the forward step's
+// geometry (64 x 4 columns of kZB planes per block, periodic in x, y and z):
+//   bphi_o = bphi + dtk (Lap pb - f'(phi_n) pb),   (pibar after the step = pb)
+//   pbm    = pb + dtd_prev bphi_o                  (the next step's pb; skipped when pbm is null)
+// and the block's coupling partials into its own slot of gacc (one per
+// block of the grid).  Per site: the six neighbours of pb summed left to
+// right in the order x+, x-, y+, y-, z+, z-, then (s - 6 pb) / a^2.
+__global__ __launch_bounds__(256) void k_adj_zstep(const double *__restrict__ phin,
+                                                   const double *__restrict__ bphi,
+                                                   const double *__restrict__ pb,
+                                                   double *__restrict__ bphi_o,
+                                                   double *__restrict__ pbm, AdjCoef k, double dtk,
+                                                   double dtd_prev, double *__restrict__ gacc) {
   const int N = k.N;
-  const long long n3 = (long long)N * N * N, P = (long long)N * N;
+  const int XT = (N + 63) / 64, YT = (N + 3) / 4, ZB = kZB, ZT = (N + ZB - 1) / ZB;
+  const int b = blockIdx.x;
+  if (b >= XT * YT * ZT) return;  // whole blocks only
+  const int xt = b % XT, yt = (b / XT) % YT, zt = b / (XT * YT);
+  const int x = xt * 64 + threadIdx.x, y = yt * 4 + threadIdx.y;
+  const bool active = x < N && y < N;
   double c0 = 0, c1 = 0, c2 = 0;
-  for (long long i = (long long)blockIdx.x * kT + threadIdx.x; i < n3; i += (long long)gridDim.x * kT) {
-    const int x = (int)(i % N), y = (int)((i / N) % N), z = (int)(i / P);
-    const double pb = bpi[i] + dtd * bphi[i];
-    double s = 0;
-    for (int d = 0; d < 3; d++)
-      for (int sg = 1; sg >= -1; sg -= 2) {
-        const long long j = nb_idx(i, x, y, z, N, d, sg);
-        s += bpi[j] + dtd * bphi[j];
-      }
-    const double f = phin_[i], q = f * f;
-    const double fp = k.c2 + 3.0 * k.c4 * q + 5.0 * k.c6 * q * q;
-    bphi_o[i] = bphi[i] + dtk * ((s - 6.0 * pb) / k.a2 - fp * pb);
-    bpi_o[i] = pb;
-    c0 -= dtk * pb * f;
-    c1 -= dtk * pb * q * f;
-    c2 -= dtk * pb * q * q * f;
+  if (active) {
+    const int z0 = zt * ZB, z1 = min(z0 + ZB, N);
+    const int xp = x + 1 == N ? 0 : x + 1, xm = x == 0 ? N - 1 : x - 1;
+    const int yp = y + 1 == N ? 0 : y + 1, ym = y == 0 ? N - 1 : y - 1;
+    const long long P = (long long)N * N;
+    const int row = y * N;
+    double bm = pb[(long long)(z0 == 0 ? N - 1 : z0 - 1) * P + row + x];
+    double bc = pb[(long long)z0 * P + row + x];
+    for (int z = z0; z < z1; z++) {
+      const long long base = (long long)z * P;
+      const double bp = pb[(long long)(z + 1 == N ? 0 : z + 1) * P + row + x];
+      double s = pb[base + row + xp] + pb[base + row + xm];
+      s += pb[base + yp * N + x];
+      s += pb[base + ym * N + x];
+      s += bp;
+      s += bm;
+      const double f = phin[base + row + x], q = f * f;
+      const double fd = k.c2 + 3.0 * k.c4 * q + 5.0 * k.c6 * q * q;
+      const double o = bphi[base + row + x] + dtk * ((s - 6.0 * bc) / k.a2 - fd * bc);
+      bphi_o[base + row + x] = o;
+      if (pbm) pbm[base + row + x] = bc + dtd_prev * o;
+      c0 -= dtk * bc * f;
+      c1 -= dtk * bc * q * f;
+      c2 -= dtk * bc * q * q * f;
+      bm = bc;
+      bc = bp;
+    }
   }
-  __shared__ double sh[kT][3];
-  sh[threadIdx.x][0] = c0; sh[threadIdx.x][1] = c1; sh[threadIdx.x][2] = c2;
+  __shared__ double sh[256][3];
+  const int t = threadIdx.y * 64 + threadIdx.x;
+  sh[t][0] = c0;
+  sh[t][1] = c1;
+  sh[t][2] = c2;
   __syncthreads();
-  for (int o = kT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o)
-      for (int j = 0; j < 3; j++) sh[threadIdx.x][j] += sh[threadIdx.x + o][j];
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int j = 0; j < 3; j++) sh[t][j] += sh[t + o][j];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int j = 0; j < 3; j++) gacc[blockIdx.x * 3 + j] += sh[0][j];
+  if (t == 0)
+    for (int j = 0; j < 3; j++) gacc[(long long)b * 3 + j] += sh[0][j];
+}
+
+// pb for the last step: bpi + dtd bphi
+__global__ void k_adj_pb(const double *__restrict__ bphi, const double *__restrict__ bpi, double dtd,
+                         long long n3, double *__restrict__ pb) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (long long)gridDim.x * blockDim.x)
+    pb[i] = bpi[i] + dtd * bphi[i];
 }
 
 __global__ void k_zero(double *p, long long n) {
@@ -356,66 +395,87 @@ extern "C" int pn_adjoint(const pn_adj_params *p, const double *phi0, const doub
   const int N = p->N, K = p->checkpoint_every;
   const size_t n3 = (size_t)N * N * N;
   const int nseg = (nsteps + K - 1) / K;
+  // K >= nsteps: every phi_n (n < nsteps) kept from the forward pass, no
+  // checkpoints and no recompute
+  const bool resident = nsteps > 0 && K >= nsteps;
   // checkpoints of (phi, pi) at steps 0, K, 2K, ...; a K-plane store of phi_n
-  double *ck = r.alloc(n3 * 2 * std::max(nseg, 1));
+  double *ck = resident ? nullptr : r.alloc(n3 * 2 * std::max(nseg, 1));
   double *store = r.alloc(n3 * (size_t)std::min(K, std::max(nsteps, 1)));
   double *phi[2] = {r.alloc(n3), r.alloc(n3)}, *pi = r.alloc(n3);
   double *rho = r.alloc(n3), *g = r.alloc(n3);
   double *bphi[2] = {r.alloc(n3), r.alloc(n3)}, *bpi[2] = {r.alloc(n3), r.alloc(n3)};
-  double *part = r.alloc(kBlocks * 6), *out3 = r.alloc(3), *gacc = r.alloc(kBlocks * 3);
-  if (!ck || !store || !phi[0] || !phi[1] || !pi || !rho || !g || !bphi[0] || !bphi[1] || !bpi[0] ||
-      !bpi[1] || !part || !out3 || !gacc)
+  const int pb = poly_blocks(N);  // blocks of the step kernels, one coupling slot each
+  double *part = r.alloc(kBlocks * 6), *out3 = r.alloc(3), *gacc = r.alloc((size_t)pb * 3);
+  if ((!resident && !ck) || !store || !phi[0] || !phi[1] || !pi || !rho || !g || !bphi[0] || !bphi[1] ||
+      !bpi[0] || !bpi[1] || !part || !out3 || !gacc)
     return afail(PN_ERR_NOMEM, "pn_adjoint: device allocation");
   hipEvent_t e0, e1, e2;
   AD_HIP(hipEventCreate(&e0));
   AD_HIP(hipEventCreate(&e1));
   AD_HIP(hipEventCreate(&e2));
-  AD_HIP(hipMemcpyAsync(phi[0], phi0, n3 * 8, hipMemcpyHostToDevice, r.s));
+  AD_HIP(hipMemcpyAsync(resident ? store : phi[0], phi0, n3 * 8, hipMemcpyHostToDevice, r.s));
   AD_HIP(hipMemcpyAsync(pi, pi0, n3 * 8, hipMemcpyHostToDevice, r.s));
   AD_HIP(hipStreamSynchronize(r.s));
-  // forward with checkpoints
+  // forward with checkpoints, or through the store
   AD_HIP(hipEventRecord(e0, r.s));
   int cur = 0;
-  const int pb = poly_blocks(N);
-  for (int n = 0; n < nsteps; n++) {
-    if (n % K == 0) {
-      double *c = ck + (size_t)(n / K) * 2 * n3;
-      AD_HIP(hipMemcpyAsync(c, phi[cur], n3 * 8, hipMemcpyDeviceToDevice, r.s));
-      AD_HIP(hipMemcpyAsync(c + n3, pi, n3 * 8, hipMemcpyDeviceToDevice, r.s));
+  if (resident) {
+    for (int n = 0; n < nsteps; n++) {
+      double *dst = n + 1 < nsteps ? store + (size_t)(n + 1) * n3 : phi[0];
+      hipLaunchKernelGGL(k_poly_step, dim3(pb), dim3(64, 4), 0, r.s, store + (size_t)n * n3, pi, dst, k, dtk[n],
+                         dtd[n]);
     }
-    hipLaunchKernelGGL(k_poly_step, dim3(pb), dim3(64, 4), 0, r.s, phi[cur], pi, phi[cur ^ 1], k, dtk[n], dtd[n]);
-    cur ^= 1;
+  } else {
+    for (int n = 0; n < nsteps; n++) {
+      if (n % K == 0) {
+        double *c = ck + (size_t)(n / K) * 2 * n3;
+        AD_HIP(hipMemcpyAsync(c, phi[cur], n3 * 8, hipMemcpyDeviceToDevice, r.s));
+        AD_HIP(hipMemcpyAsync(c + n3, pi, n3 * 8, hipMemcpyDeviceToDevice, r.s));
+      }
+      hipLaunchKernelGGL(k_poly_step, dim3(pb), dim3(64, 4), 0, r.s, phi[cur], pi, phi[cur ^ 1], k, dtk[n], dtd[n]);
+      cur ^= 1;
+    }
   }
   AD_HIP(hipGetLastError());
   double F = 0, cdir[3] = {0, 0, 0};
   rc = observable(r, k, phi[cur], pi, rho, part, out3, &F, g, bphi[0], bpi[0], cdir);
   if (rc != PN_OK) return rc;
   AD_HIP(hipEventRecord(e1, r.s));
-  // backward: segments in reverse
-  hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, r.s, gacc, (long long)kBlocks * 3);
-  int bc = 0;
+  // backward: segments in reverse.  bphi[bc] holds phibar; bpi[pc] holds pb
+  // of the step about to run (after the last one: pibar of step 0)
+  hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, r.s, gacc, (long long)pb * 3);
+  int bc = 0, pc = 0;
+  if (nsteps > 0) {
+    hipLaunchKernelGGL(k_adj_pb, dim3(kBlocks), dim3(kT), 0, r.s, bphi[0], bpi[0], dtd[nsteps - 1], (long long)n3,
+                       bpi[1]);
+    pc = 1;
+  }
   for (int sg = nseg - 1; sg >= 0; sg--) {
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K);
-    const double *c = ck + (size_t)sg * 2 * n3;
-    // recompute phi_{n0..n1-1} into the store; pi evolves in the scratch pi
-    AD_HIP(hipMemcpyAsync(store, c, n3 * 8, hipMemcpyDeviceToDevice, r.s));
-    AD_HIP(hipMemcpyAsync(pi, c + n3, n3 * 8, hipMemcpyDeviceToDevice, r.s));
-    for (int n = n0; n < n1 - 1; n++) {
-      double *src = store + (size_t)(n - n0) * n3, *dst = store + (size_t)(n - n0 + 1) * n3;
-      hipLaunchKernelGGL(k_poly_step, dim3(pb), dim3(64, 4), 0, r.s, src, pi, dst, k, dtk[n], dtd[n]);
+    if (!resident) {
+      // recompute phi_{n0..n1-1} into the store; pi evolves in the scratch pi
+      const double *c = ck + (size_t)sg * 2 * n3;
+      AD_HIP(hipMemcpyAsync(store, c, n3 * 8, hipMemcpyDeviceToDevice, r.s));
+      AD_HIP(hipMemcpyAsync(pi, c + n3, n3 * 8, hipMemcpyDeviceToDevice, r.s));
+      for (int n = n0; n < n1 - 1; n++) {
+        double *src = store + (size_t)(n - n0) * n3, *dst = store + (size_t)(n - n0 + 1) * n3;
+        hipLaunchKernelGGL(k_poly_step, dim3(pb), dim3(64, 4), 0, r.s, src, pi, dst, k, dtk[n], dtd[n]);
+      }
     }
     for (int n = n1 - 1; n >= n0; n--) {
-      hipLaunchKernelGGL(k_adj_step, dim3(kBlocks), dim3(kT), 0, r.s, store + (size_t)(n - n0) * n3,
-                         bphi[bc], bpi[bc], bphi[bc ^ 1], bpi[bc ^ 1], k, dtk[n], dtd[n], gacc);
+      double *next = n > 0 ? bpi[pc ^ 1] : nullptr;
+      hipLaunchKernelGGL(k_adj_zstep, dim3(pb), dim3(64, 4), 0, r.s, store + (size_t)(n - n0) * n3, bphi[bc],
+                         bpi[pc], bphi[bc ^ 1], next, k, dtk[n], n > 0 ? dtd[n - 1] : 0.0, gacc);
       bc ^= 1;
+      if (n > 0) pc ^= 1;
     }
     AD_HIP(hipGetLastError());
   }
   AD_HIP(hipEventRecord(e2, r.s));
-  std::vector<double> ga((size_t)kBlocks * 3);
+  std::vector<double> ga((size_t)pb * 3);
   AD_HIP(hipMemcpyAsync(ga.data(), gacc, ga.size() * 8, hipMemcpyDeviceToHost, r.s));
   if (grad_phi0) AD_HIP(hipMemcpyAsync(grad_phi0, bphi[bc], n3 * 8, hipMemcpyDeviceToHost, r.s));
-  if (grad_pi0) AD_HIP(hipMemcpyAsync(grad_pi0, bpi[bc], n3 * 8, hipMemcpyDeviceToHost, r.s));
+  if (grad_pi0) AD_HIP(hipMemcpyAsync(grad_pi0, bpi[pc], n3 * 8, hipMemcpyDeviceToHost, r.s));
   AD_HIP(hipStreamSynchronize(r.s));
   float fms = 0, bms = 0;
   AD_HIP(hipEventElapsedTime(&fms, e0, e1));
@@ -425,7 +485,7 @@ extern "C" int pn_adjoint(const pn_adj_params *p, const double *phi0, const doub
   (void)hipEventDestroy(e2);
   for (int j = 0; j < 3; j++) {
     double hi = cdir[j], lo = 0;
-    for (int b = 0; b < kBlocks; b++) {
+    for (int b = 0; b < pb; b++) {
       const double x = ga[(size_t)b * 3 + j], s = hi + x, bp = s - hi;
       lo += (hi - (s - bp)) + (x - bp);
       hi = s;

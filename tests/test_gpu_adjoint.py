@@ -52,7 +52,24 @@ def test_gradients_match_finite_differences_and_tlm():
         assert abs((fp - fm) / (2 * eps) - gc[j]) <= 1e-6 * abs(gc[j])
 
 
+def test_resident_store_equals_checkpoints():
+    # checkpoint_every >= nsteps keeps every phi_n from the forward pass (no
+    # recompute); the same kernels in the same order otherwise, so F and
+    # every gradient are bitwise those of checkpointed runs (K = 7 and 1)
+    N, S = 12, 30
+    phi0, pi0 = _state(N, 31)
+    dtk, dtd, _ = pn.time_factors(0.0, 0.1, 0.5, S)
+    runs = [pn.adjoint(pn.adj_params(N, checkpoint_every=K), phi0, pi0, dtk, dtd) for K in (S, 100, 7, 1)]
+    F0, gphi0, gpi0, gc0, r0 = runs[0]
+    for F, gphi, gpi, gc, r in runs[1:]:
+        assert F == F0
+        assert np.array_equal(gphi, gphi0) and np.array_equal(gpi, gpi0) and np.array_equal(gc, gc0)
+    # the resident run holds S planes of phi_n and no checkpoints
+    assert r0.peak_bytes > runs[2][4].peak_bytes
+
+
 def test_torch_autograd_harness():
+    pn.lib()  # the library first: it refuses a process whose HIP runtime came from torch's wheel
     torch = pytest.importorskip("torch")
 
     class OscillonFraction(torch.autograd.Function):
