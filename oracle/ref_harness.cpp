@@ -18,12 +18,15 @@
 // (at_time, peak pos, Peak_value, avPhi, V, PotEn, KinEn, GradEn).
 // With REF_SAVECONFIG=n in the environment the final state is also written by
 // the reference's SaveConfig(n) (./field-n-0.out, ./moment-n.out).
+// ref_harness --info - <parameters>: the reference's Print_Info_Console (to
+// stdout) and Print_Info_File (./output/0/info.txt) for those parameters.
 #define MPICH_SKIP_MPICXX 1
 #include "islands.cpp"
 #include "renormalization.cpp"
 #include <sys/stat.h>
 
 #include <cstdio>
+#include <string>
 #include <vector>
 
 static void die(const char *m) { std::fprintf(stderr, "ref_harness: %s\n", m); std::exit(2); }
@@ -36,6 +39,14 @@ int main(int argc, char **argv) {
   // path sits where the program name would be
   GetParameters(argc - 2, argv + 2);
   CompleteParameters();
+  if (std::string(argv[1]) == "--info") {
+    mkdir("output", 0755);
+    mkdir("output/0", 0755);  // LASTRUN is 0
+    Print_Info_Console();
+    Print_Info_File();
+    MPI_Finalize();
+    return 0;
+  }
   LocalLattice::init();
   fix_tadpole();
 

@@ -316,13 +316,50 @@ void Prepare_Output() {  // fieldanalysis.cpp:48-79
   mkdir(folder, 0755);
 }
 
-void Print_Info_Console() {  // fieldanalysis.cpp:81-92 (theory 0)
+// The parameter summary of Print_Info_Console and Print_Info_File for theory
+// 0 and initialize 3 (fieldanalysis.cpp:81-158,160-247).  The two differ in
+// one line: the console prints the time step a_t*dtBaseTrans as "dt", the
+// file the two base factors as "dtBT" and "dtBL".
+static void write_info(std::ostream &out, bool file) {
+  out << "Parameters" << std::endl;
+  out << "Nc=" << Nc << std::endl;
+  out << "m^2=" << Mass_Mode * M_sqr << std::endl;
+  out << "Lambda=" << Lambda_Mode * Lambda << std::endl;
+  out << "Nyu=" << (Lambda / Kappa) * (Lambda / Kappa) << std::endl;
+  out << "N_s=" << N_t << std::endl;
+  out << "a_s=" << a_t << std::endl;
+  out << "T0=" << T0 << std::endl;
+  out << "TMax=" << TMax << std::endl;
+  if (file) {
+    out << "dtBT=" << dtBaseTrans << std::endl;
+    out << "dtBL=" << dtBaseLong << std::endl;
+  } else {
+    out << "dt=" << a_t * dtBaseTrans << std::endl;
+  }
+  out << "TSO=" << TSO << std::endl;
+  out << "TMO=" << TMO << std::endl;
+  out << "f0(p)=" << Amplitude << "/Lambda, for p<" << QS << std::endl;
+  out << "phi0_0=" << phi0_0 << std::endl;
+  out << "phi0_1=" << phi0_1 << std::endl;
+  out << "pi0_0=" << pi0_0 << std::endl;
+  out << "pi0_1=" << pi0_1 << std::endl;
+  out << std::endl << runs;
+  out << (runs == 1 ? " run with" : " runs with");
+  out << (renorm_tadpole == 1 ? " " : "out ");
+  out << "renormalization." << std::endl;
+}
+
+void Print_Info_Console() {  // fieldanalysis.cpp:81-158
   if (ID != 0) return;
-  std::cout << "Parameters" << std::endl;
-  std::cout << "Nc=" << Nc << std::endl;
-  std::cout << "m^2=" << Mass_Mode * M_sqr << std::endl;
-  std::cout << "Lambda=" << Lambda_Mode * Lambda << std::endl;
-  std::cout << "Nyu=" << (Lambda / Kappa) * (Lambda / Kappa) << std::endl;
+  write_info(std::cout, false);
+}
+
+void Print_Info_File() {  // fieldanalysis.cpp:160-247: ./output/<LASTRUN>/info.txt
+  if (ID != 0 || !write_output) return;
+  char fname[256];
+  std::snprintf(fname, sizeof fname, "./output/%i/info.txt", LASTRUN);
+  std::ofstream out(fname);
+  write_info(out, true);
 }
 
 void Print_AveragesExtrema() {  // fieldanalysis.cpp:748-799

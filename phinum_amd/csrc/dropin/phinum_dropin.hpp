@@ -90,6 +90,7 @@ void CalculateVolumeAverages();
 void CalculateExtrema();
 void Prepare_Output();
 void Print_Info_Console();
+void Print_Info_File();
 void Print_AveragesExtrema();
 bool false_vacuum_decayed();
 void SaveConfig(int Name);     // fieldanalysis.cpp:252-282

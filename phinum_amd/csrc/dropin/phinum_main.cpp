@@ -18,6 +18,7 @@ int main(int argc, char **argv) {
     CompleteParameters();
     Prepare_Output();
     Print_Info_Console();
+    Print_Info_File();
     LocalLattice::init();
     MomentumSpace::init();
     for (runID = 1; runID <= runs; runID++) {

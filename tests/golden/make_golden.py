@@ -4,7 +4,8 @@ hot-path sources, compiled unmodified with strict IEEE flags
 exists).  The fixtures are data only: seeded inputs and the reference's
 outputs for them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          (everything)
+    python tests/golden/make_golden.py info     (the parameter summary only)
 """
 import os
 import sys
@@ -98,6 +99,36 @@ def main_config():
     print("wrote saveconfig_n8_s40")
 
 
+# the drop-in driver's parameters in tests/test_gpu_dropin.py
+def driver_info_args():
+    N = 16
+    m = O.model(N)
+    return ["-runs", "1", "-Nt", str(N), "-at", "0.5", "-M", "1", "-Mmode", "1", "-L", "6", "-Lmode", "-1",
+            "-Kappa", "%.17g" % m["kappa"], "-T0", "0", "-TMax", "2.0", "-dt", "0.1", "-TSO", "0.5", "-TMO", "1.0",
+            "-Noise", "0.5", "-Ampl", "0", "-QS", "%.17g" % (3.0 ** 0.5 / 0.5), "-Q2", "%.17g" % (3.0 ** 0.5 / 0.5),
+            "-phi0_0", "%.17g" % 6.0 ** 0.5, "-seed", "1234", "-islands", "1"]
+
+
+def main_info():
+    """Print_Info_Console (stdout) and Print_Info_File (info.txt) of the
+    reference for the drop-in driver test's parameters (fieldanalysis.cpp:81-247)."""
+    import json
+    import subprocess
+    import tempfile
+    d = os.path.join(HERE, "info")
+    os.makedirs(d, exist_ok=True)
+    args = driver_info_args()
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([O.REF_HARNESS, "--info", "-"] + args, cwd=td, capture_output=True, text=True,
+                           timeout=60)
+        if r.returncode != 0:
+            raise SystemExit(r.stderr)
+        info = open(os.path.join(td, "output", "0", "info.txt")).read()
+    with open(os.path.join(d, "driver_info.json"), "w") as f:
+        json.dump({"args": args, "console": r.stdout, "info_txt": info}, f, indent=1)
+    print("wrote info/driver_info.json")
+
+
 def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref/ref_harness is not built (needs the reference tree)")
@@ -116,6 +147,10 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
-    main_track()
-    main_config()
+    if sys.argv[1:] == ["info"]:
+        main_info()
+    else:
+        main()
+        main_track()
+        main_config()
+        main_info()

@@ -1,5 +1,6 @@
 """The reference-named host layer and driver (phinum_main, main.cpp:21-117)
 against the oracle, plus SetInitialConditions on the device."""
+import json
 import os
 import subprocess
 
@@ -11,6 +12,7 @@ import phinum_amd as pn
 
 pytestmark = pytest.mark.gpu
 MAIN = os.path.join(os.path.dirname(pn.__file__), "phinum_main")
+INFO = os.path.join(os.path.dirname(__file__), "golden", "info", "driver_info.json")
 
 
 @pytest.mark.parametrize("N,seed,prec", [(16, 1234, pn.PN_FP64), (32, 7, pn.PN_FP64),
@@ -189,3 +191,14 @@ def test_driver_queued_steps_bitwise(tmp_path):
     assert steps > 256
     ophi, opi = O.evolve(m, phi, pi, steps)
     assert (tmp_path / "field-9-0.out").read_text() == O.config_text(ophi, opi)
+
+
+def test_driver_info_matches_reference(tmp_path):
+    # Print_Info_Console and Print_Info_File (fieldanalysis.cpp:81-247): the
+    # reference's own text for the same parameters (the fixture comes from
+    # oracle/_ref/ref_harness --info), byte for byte
+    g = json.load(open(INFO))
+    r = subprocess.run([MAIN] + g["args"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert g["console"] in r.stdout
+    assert (tmp_path / "output" / "1" / "info.txt").read_text() == g["info_txt"]
