@@ -45,6 +45,29 @@ def test_dot_product_tlm_vs_adjoint():
         assert abs(tl - ad) <= 1e-12 * max(abs(tl), 1e-300) + 1e-15
 
 
+def test_dot_product_tlm_vs_adjoint_128_cube():
+    # the same identity at the adjoint configuration's lattice, 128^3, over
+    # 100 steps (the linearisation of dynamics.cpp:52-194 against the
+    # checker's reverse pass): 7.6e-14 relative here.  A sign error in the
+    # coupling gradient, or a gradient for phi0 that misses pi0's share,
+    # breaks it by orders of magnitude more than the tolerance.
+    N, S = 128, 100
+    p = O.adj_params(N)
+    rng = np.random.default_rng(128)
+    phi0 = 1.0 + 0.6 * rng.standard_normal(N ** 3)
+    pi0 = 0.3 * rng.standard_normal(N ** 3)
+    dtk, dtd, _ = O.time_factors(S)
+    F, gphi, gpi, gc = O.adj_reverse(p, phi0, pi0, dtk, dtd)
+    v, w, dc = rng.standard_normal(N ** 3), rng.standard_normal(N ** 3), rng.standard_normal(3)
+    tl = O.adj_tlm(p, phi0, pi0, dtk, dtd, v, w, dc)
+    ad = float(np.dot(gphi, v) + np.dot(gpi, w) + np.dot(gc, dc))
+    assert abs(tl - ad) <= 1e-11 * abs(tl)
+    wrong = (float(np.dot(gphi, v) + np.dot(gpi, w) - np.dot(gc, dc)),
+             float(np.dot(gphi, v) + np.dot(gc, dc)))
+    for bad in wrong:
+        assert abs(tl - bad) > 1e3 * 1e-11 * abs(tl)
+
+
 @pytest.mark.parametrize("which", ["phi", "c4", "c6"])
 def test_adjoint_matches_central_differences(which):
     N, S = 6, 40
