@@ -127,6 +127,26 @@ def main_info():
     with open(os.path.join(d, "driver_info.json"), "w") as f:
         json.dump({"args": args, "console": r.stdout, "info_txt": info}, f, indent=1)
     print("wrote info/driver_info.json")
+    # command-line edge cases of the -key value parser (cfile.c:55-73,252-278):
+    # a later duplicate, an unknown key, a key without a value at the end,
+    # exponent notation, a negative value first, an over-long key, a
+    # fractional value for an integer key, no arguments at all
+    cases = {"duplicate": args + ["-Nt", "24"], "unknown_key": args + ["-foo", "3"],
+             "key_without_value": args + ["-TSO"], "exponent": args + ["-TMax", "1e3"],
+             "negative_first": ["-Lmode", "-1"] + args, "long_key": args + ["-" + "x" * 100, "5"],
+             "fraction_for_int": args + ["-Nt", "24.7"], "no_arguments": []}
+    out = {}
+    for name, a in cases.items():
+        with tempfile.TemporaryDirectory() as td:
+            r = subprocess.run([O.REF_HARNESS, "--info", "-"] + a, cwd=td, capture_output=True, text=True,
+                               timeout=60)
+            if r.returncode != 0:
+                raise SystemExit(r.stderr)
+            out[name] = {"args": a, "console": r.stdout,
+                         "info_txt": open(os.path.join(td, "output", "0", "info.txt")).read()}
+    with open(os.path.join(d, "parse_cases.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote info/parse_cases.json")
 
 
 def main():

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 PKG = os.path.join(ROOT, "phinum_amd")
 INFO = os.path.join(HERE, "golden", "info", "driver_info.json")
+CASES = os.path.join(HERE, "golden", "info", "parse_cases.json")
 
 
 @pytest.fixture(scope="module")
@@ -50,3 +51,16 @@ def test_parameter_summary_sees_each_parameter(checker, tmp_path):
     assert "N_s=24\n" in r.stdout and "dt=0.1\n" in r.stdout and "phi0_0=1.5\n" in r.stdout
     info = (tmp_path / "output" / "0" / "info.txt").read_text()
     assert "dtBT=0.2\ndtBL=0.2\n" in info and r.stdout != g["console"]
+
+
+@pytest.mark.parametrize("name", sorted(json.load(open(CASES))))
+def test_parameter_parsing_edge_cases_match_reference(checker, tmp_path, name):
+    # the reference's own summary for command lines with a duplicate key, an
+    # unknown key, a trailing key without a value, exponent notation, a
+    # negative value first, an over-long key, a fraction for an integer, and
+    # no arguments (fixture from oracle/_ref/ref_harness --info)
+    c = json.load(open(CASES))[name]
+    r = subprocess.run([checker] + c["args"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == c["console"]
+    assert (tmp_path / "output" / "0" / "info.txt").read_text() == c["info_txt"]
